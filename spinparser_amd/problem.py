@@ -1,0 +1,142 @@
+"""SU(2) flow problem descriptor: the flattened tables the flow kernels consume.
+
+Everything the right-hand side needs besides the vertex state, derived once from
+the task file and the lattice factory:
+
+* ``mesh``              positive float32 mesh (``FrequencyDiscretization``)
+* ``cutoffs``           float32 cutoff schedule (``CutoffDiscretization``)
+* ``site_rid`` / ``inv_rid``   ``getSites()`` / ``getInvertedSites()`` rids
+  (``src/Lattice.hpp:481-494``), used by the bundled vertex gathers
+* ``ovl_off/ovl_rid1/ovl_rid2``  overlap lists as CSR (``src/Lattice.hpp:470-474``)
+* ``range_fwd/range_inv``  ``symmetryTransform(0, j)`` / ``(j, 0)`` for j in the
+  range of the origin (self-energy flow, ``src/SU2/SU2FrgCore.cpp:151-155``)
+* ``corr_rid``          ``symmetryTransform(b, j)`` for the correlation output
+  (``src/SU2/SU2MeasurementCorrelation.cpp:161-176``)
+* ``spin``, ``normalization`` core options (``src/SU2/SU2FrgCore.cpp:19-29``)
+* ``coupling_zz``       per-rid sum of J_zz entering the initial condition
+  (``src/SU2/SU2EffectiveAction.hpp:46-59``)
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import numpy as np
+
+from .lattice import Lattice, string_to_float
+from .task import Task, parse_task
+
+
+class CoreOptionError(ValueError):
+    """Unknown core option (``Exception::Type::InitializationError`` in the reference)."""
+
+
+@dataclass
+class SU2Problem:
+    mesh: np.ndarray
+    cutoffs: np.ndarray
+    L: int
+    site_rid: np.ndarray
+    inv_rid: np.ndarray
+    ovl_off: np.ndarray
+    ovl_rid1: np.ndarray
+    ovl_rid2: np.ndarray
+    range_fwd: np.ndarray
+    range_inv: np.ndarray
+    B: int
+    Rb: int
+    corr_rid: np.ndarray
+    spin: np.float32
+    normalization: np.float32
+    coupling_zz_f32: np.ndarray   # (L,) float32 init values, accumulated in float like the reference
+    coupling_zz_f64: np.ndarray   # (L,) float64 init values
+    lattice: Optional[Lattice] = None
+
+    @property
+    def nw(self) -> int:
+        return int(self.mesh.shape[0])
+
+    @property
+    def n_freq(self) -> int:
+        """Number of frequency triples N_f = N_w^2 (N_w + 1) / 2."""
+        n = self.nw
+        return n * n * (n + 1) // 2
+
+    @property
+    def vertex_size(self) -> int:
+        return self.n_freq * self.L
+
+    @property
+    def overlap_total(self) -> int:
+        return int(self.ovl_off[-1])
+
+    def initial_state(self, dtype=np.float64):
+        """(v2, v4s, v4d) at the first cutoff: Gamma_s(rid) = J_zz(0->rid)/norm, the rest 0."""
+        v2 = np.zeros(self.nw, dtype=dtype)
+        per_rid = self.coupling_zz_f32 if dtype == np.float32 else self.coupling_zz_f64
+        v4s = np.tile(per_rid.astype(dtype), self.n_freq)
+        v4d = np.zeros(self.vertex_size, dtype=dtype)
+        return v2, v4s, v4d
+
+    def flops_per_evaluation(self, calls_s: int, calls_t: int, calls_u: int) -> float:
+        """Algorithmic FLOP of one dGamma evaluation from integrand-call counts:
+        94 L per S/U kernel call, 106 L + 4 O per T call, + Euler 2 |Gamma|."""
+        L, O = self.L, self.overlap_total
+        return (94.0 * L * (calls_s + calls_u) + (106.0 * L + 4.0 * O) * calls_t
+                + 2.0 * 2 * self.vertex_size + 2.0 * self.nw)
+
+
+def su2_problem(task: Task, resource_path: Optional[str] = None) -> SU2Problem:
+    lattice, model, core_options = task.build_lattice(resource_path)
+    return su2_problem_from_lattice(task.frequencies, task.cutoffs, lattice, core_options)
+
+
+def su2_problem_from_lattice(mesh: np.ndarray, cutoffs: np.ndarray, lattice: Lattice,
+                             core_options: Dict[str, str]) -> SU2Problem:
+    spin = np.float32(0.5)
+    normalization = np.float32(np.nan)
+    for k, v in core_options.items():
+        if k == "spin":
+            spin = string_to_float(v)
+        elif k == "normalization":
+            normalization = string_to_float(v)
+        elif k == "backend":
+            continue  # selector for the GPU core, consumed by the factory
+        else:
+            raise CoreOptionError("Unknown spin model option '%s'." % k)
+    if np.isnan(normalization):
+        normalization = np.float32(np.float32(2.0) * spin)
+
+    L = lattice.size
+    zz32 = np.zeros(L, dtype=np.float32)
+    zz64 = np.zeros(L, dtype=np.float64)
+    for rid, strength in lattice.couplings:
+        zz32[rid] = np.float32(zz32[rid] + np.float32(strength[0, 0] / normalization))
+        zz64[rid] += float(strength[0, 0]) / float(normalization)
+
+    off, rid1, rid2 = lattice.overlap_csr()
+    r0 = lattice.ranges[0]
+    range_fwd = np.array([lattice.sym_rid[0, j] for j in r0], dtype=np.int32)
+    range_inv = np.array([lattice.sym_rid[j, 0] for j in r0], dtype=np.int32)
+    B = len(lattice.basis)
+    Rb = len(r0)
+    corr = np.zeros(B * Rb, dtype=np.int32)
+    for b in range(B):
+        rb = lattice.ranges[b]
+        if len(rb) != Rb:
+            raise ValueError("basis sites with ranges of different size are not supported")
+        for k, j in enumerate(rb):
+            corr[b * Rb + k] = lattice.sym_rid[lattice.basis_ids[b], j]
+    if (range_fwd < 0).any() or (range_inv < 0).any() or (corr < 0).any():
+        raise ValueError("incomplete symmetry table")
+
+    return SU2Problem(mesh=np.ascontiguousarray(mesh, dtype=np.float32),
+                      cutoffs=np.ascontiguousarray(cutoffs, dtype=np.float32), L=L,
+                      site_rid=lattice.site_rid.astype(np.int32), inv_rid=lattice.inverted_site_rid.astype(np.int32),
+                      ovl_off=off, ovl_rid1=rid1, ovl_rid2=rid2, range_fwd=range_fwd, range_inv=range_inv,
+                      B=B, Rb=Rb, corr_rid=corr, spin=np.float32(spin), normalization=np.float32(normalization),
+                      coupling_zz_f32=zz32, coupling_zz_f64=zz64, lattice=lattice)
+
+
+def su2_problem_from_xml(text_or_path: str, is_text: bool = False, resource_path: Optional[str] = None) -> SU2Problem:
+    return su2_problem(parse_task(text_or_path, is_text=is_text), resource_path)
