@@ -1,0 +1,126 @@
+/*
+ * spfrg.h -- C ABI of the MI355X pf-FRG flow engine (SU(2) core).
+ *
+ * One context holds one flow: the vertex state (self-energy Sigma and the
+ * two-particle vertex Gamma in its spin and density channels), the flow
+ * (dSigma/dLambda, dGamma/dLambda), the lattice tables and all scratch, resident
+ * in device memory.  The entry points are the operations the reference's FrgCore
+ * plugin boundary performs on its cores; each cites the reference interface it
+ * replaces (paths relative to the SpinParser repository).
+ *
+ * Conventions
+ *   - Every call returns an int status: SP_OK (0) or one of the SP_E* codes;
+ *     sp_last_error() returns a message for the last failure on a context.
+ *     Nothing throws across the ABI.
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - Vertex arrays use the reference layout: Gamma[(s(s+1)/2 + u) * Nw * L +
+ *     t * L + rid] with s >= u (src/SU2/SU2VertexTwoParticle.hpp:595-605), one
+ *     array per channel (spin "ss", density "dd"); Sigma[Nw].
+ *   - Cutoff values are the reference's float grid values passed as double.
+ *   - A context is used from one host thread at a time; its work runs on the
+ *     context's own HIP stream (default priority) on the current device.
+ */
+#ifndef SPFRG_H
+#define SPFRG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_OK 0
+#define SP_EARG 1      /* invalid argument or unsupported problem size        */
+#define SP_ENOMEM 2    /* device or host allocation failed                     */
+#define SP_EHIP 3      /* HIP runtime error                                    */
+#define SP_ESTATE 4    /* call out of order (e.g. flow read before a step)     */
+
+#define SP_PRECISION_FP64 0   /* plain FP64 form                              */
+
+/* Problem descriptor: the flattened lattice / mesh tables of one SU(2) flow.
+ * Built host-side by the lattice factory; replaces what the reference reads
+ * through FrgCommon::frequency() / FrgCommon::lattice()
+ * (src/FrequencyDiscretization.hpp:164-361, src/Lattice.hpp:297-552). */
+typedef struct sp_su2_problem {
+  int nw;                 /* positive Matsubara frequencies N_w               */
+  const float *mesh;      /* N_w ascending positive mesh points (float grid)  */
+  int L;                  /* representative lattice sites                      */
+  const int *site_rid;    /* L: getSites()[j].rid       (Lattice.hpp:491)     */
+  const int *inv_rid;     /* L: getInvertedSites()[j].rid (Lattice.hpp:481)   */
+  const int *ovl_off;     /* L+1: CSR offsets of getOverlap(rid) (Lattice.hpp:470) */
+  const int *ovl_rid1;    /* ovl_off[L]: overlap rid1                          */
+  const int *ovl_rid2;    /* ovl_off[L]: overlap rid2                          */
+  int R;                  /* sites in range of the origin                      */
+  const int *range_fwd;   /* R: symmetryTransform(0, j), j in getRange(0)      */
+  const int *range_inv;   /* R: symmetryTransform(j, 0)                        */
+  int B;                  /* basis sites                                       */
+  int Rb;                 /* sites in range of each basis site                 */
+  const int *corr_rid;    /* B*Rb: symmetryTransform(b, j) for the .obs output */
+  double spin;            /* spin length S (core option "spin")                */
+  int precision;          /* SP_PRECISION_FP64                                 */
+} sp_su2_problem;
+
+typedef struct sp_ctx sp_ctx;
+
+/* Create a flow context and upload the tables; replaces the construction of
+ * SU2FrgCore + SU2EffectiveAction (src/SU2/SU2FrgCore.cpp:17-81).  The state is
+ * zero until sp_set_state(). */
+int sp_ctx_create(const sp_su2_problem *problem, sp_ctx **out);
+
+/* Release every device buffer; replaces SU2FrgCore::~SU2FrgCore
+ * (src/SU2/SU2FrgCore.cpp:83-87). */
+int sp_ctx_destroy(sp_ctx *ctx);
+
+/* Upload the state at a cutoff; replaces writing EffectiveAction::cutoff and the
+ * vertex arrays, as SU2EffectiveAction's initial condition and
+ * readCheckpoint do (src/SU2/SU2EffectiveAction.hpp:38-60,159-205). */
+int sp_set_state(sp_ctx *ctx, double cutoff, const double *v2, const double *v4ss, const double *v4dd);
+
+/* Download the state; the read side of SU2EffectiveAction::writeCheckpoint
+ * (src/SU2/SU2EffectiveAction.hpp:79-150).  Null pointers skip that array. */
+int sp_get_state(sp_ctx *ctx, double *cutoff, double *v2, double *v4ss, double *v4dd);
+
+/* Download the flow of the last sp_compute_step (the reference's FrgCore::_flow,
+ * src/FrgCore.hpp:140).  Null pointers skip that array. */
+int sp_get_flow(sp_ctx *ctx, double *dv2, double *dv4ss, double *dv4dd);
+
+/* Evaluate the right-hand side at the state's cutoff: dSigma first, then dGamma
+ * with its Katanin term; replaces SU2FrgCore::computeStep
+ * (src/SU2/SU2FrgCore.cpp:89-109, 139-431).  Asynchronous on the context
+ * stream. */
+int sp_compute_step(sp_ctx *ctx);
+
+/* Explicit Euler update in place, Gamma += (new_cutoff - cutoff) * dGamma,
+ * Sigma likewise, then cutoff = new_cutoff; replaces SU2FrgCore::finalizeStep
+ * (src/SU2/SU2FrgCore.cpp:111-137).  Asynchronous. */
+int sp_finalize_step(sp_ctx *ctx, double new_cutoff);
+
+/* 1 if the flow of the last sp_compute_step holds a NaN, else 0; replaces
+ * SU2EffectiveAction::isDiverged on FrgCore::_flow
+ * (src/SU2/SU2EffectiveAction.hpp:212-230, src/SpinParser.cpp:151). */
+int sp_is_diverged(sp_ctx *ctx, int *diverged);
+
+/* Static (nu = 0) spin and density susceptibilities at the state's cutoff,
+ * B*Rb values each in the .obs site order (SU2CorZZ, SU2CorDD); replaces
+ * SU2MeasurementCorrelation::_calculateCorrelation
+ * (src/SU2/SU2MeasurementCorrelation.cpp:77-177).  Synchronous. */
+int sp_measure_correlation(sp_ctx *ctx, double *zz, double *dd);
+
+/* Wait for all work queued on the context stream. */
+int sp_synchronize(sp_ctx *ctx);
+
+/* Message of the last failed call on ctx (static storage per context). */
+const char *sp_last_error(const sp_ctx *ctx);
+
+/* Kernel timing with HIP events on the context stream (no reference
+ * counterpart; replaces LoadManager's per-stack compute-time accounting,
+ * src/lib/LoadManager.hpp:583-648).  kernel ids: 0 dGamma, 1 dSigma,
+ * 2 propagator weight tables, 3 Euler update. */
+int sp_profile_enable(sp_ctx *ctx, int enable);
+int sp_profile_get(sp_ctx *ctx, int kernel, double *total_ms, long *launches);
+
+/* Device bytes held by the context (admission by footprint). */
+int sp_ctx_footprint(const sp_su2_problem *problem, long long *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPFRG_H */

@@ -100,6 +100,14 @@ class Oracle:
           ctypes.c_int(it0), ctypes.c_int(it1), _ptr(dss, t), _ptr(ddd, t), ctypes.c_int(nthreads), _ptr(calls, _lp))
         return dss, ddd, calls
 
+    def dsigma(self, cutoff, v2, v4s, v4d):
+        """dSigma/dLambda only (FP64)."""
+        v2, v4s, v4d = (np.ascontiguousarray(x, dtype=np.float64) for x in (v2, v4s, v4d))
+        dv2 = np.zeros(self.prob.nw)
+        self.lib.or_su2_dsigma_f64(ctypes.byref(self.c), ctypes.c_double(float(cutoff)), _ptr(v2, _f64p),
+                                   _ptr(v4s, _f64p), _ptr(v4d, _f64p), _ptr(dv2, _f64p))
+        return dv2
+
     def correlation(self, cutoff, v2, v4s, v4d, dtype=np.float64):
         p = self.prob
         zz = np.zeros(p.B * p.Rb, dtype=dtype)

@@ -657,6 +657,12 @@ void or_su2_items_f32(const or_problem *p, float cutoff, const float *v2, const 
   if (calls) { calls[0] = cS; calls[1] = cT; calls[2] = cU; }
 }
 
+void or_su2_dsigma_f64(const or_problem *p, double cutoff, const double *v2, const double *ss, const double *dd,
+                       double *dv2) {
+  Ctx<double> c(*p, v2, nullptr, ss, dd);
+  for (int i = 0; i < p->nw; ++i) dv2[i] = vertex_single_particle(c, i, cutoff);
+}
+
 void or_su2_correlation_f64(const or_problem *p, double cutoff, const double *v2, const double *ss,
                             const double *dd, double *zz, double *ddo) {
   correlation<double>(*p, cutoff, v2, ss, dd, zz, ddo);

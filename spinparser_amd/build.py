@@ -1,0 +1,48 @@
+"""Build recipe for the HIP library (gfx950) -- compiled in-tree with hipcc.
+
+``python -m spinparser_amd.build`` (or ``__graft_entry__.build()``) writes
+``spinparser_amd/_lib/libspfrg.so``.  Device code is compiled with
+``-ffp-contract=off`` so that frequency arguments, window predicates and mesh
+bracket comparisons round exactly like the reference; the hot accumulations use
+explicit ``fma()``.  Packed-FP32 instructions are disabled for all device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_DIR = os.path.join(PKG, "_lib")
+LIB_PATH = os.path.join(LIB_DIR, "libspfrg.so")
+SOURCES = [os.path.join(PKG, "csrc", "su2_flow.hip")]
+
+
+def hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build_lib(verbose: bool = False) -> str:
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+           "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include"),
+           "-o", tmp] + SOURCES
+    if verbose:
+        print(" ".join(cmd))
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed for libspfrg.so")
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    print(build_lib(verbose=True))

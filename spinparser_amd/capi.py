@@ -1,0 +1,193 @@
+"""ctypes binding of the C ABI declared in ``include/spfrg.h`` (libspfrg.so).
+
+There is no fallback: if the HIP library is missing or a call fails, an
+exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+from .build import LIB_PATH
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+SP_OK, SP_EARG, SP_ENOMEM, SP_EHIP, SP_ESTATE = 0, 1, 2, 3, 4
+SP_PRECISION_FP64 = 0
+
+KERNEL_DGAMMA, KERNEL_DSIGMA, KERNEL_WEIGHTS, KERNEL_EULER = 0, 1, 2, 3
+
+EXPORTED_SYMBOLS = (
+    "sp_ctx_create", "sp_ctx_destroy", "sp_set_state", "sp_get_state", "sp_get_flow", "sp_compute_step",
+    "sp_finalize_step", "sp_is_diverged", "sp_measure_correlation", "sp_synchronize", "sp_last_error",
+    "sp_profile_enable", "sp_profile_get", "sp_ctx_footprint",
+)
+
+
+class SpSu2Problem(ctypes.Structure):
+    _fields_ = [("nw", ctypes.c_int), ("mesh", _f32p), ("L", ctypes.c_int),
+                ("site_rid", _i32p), ("inv_rid", _i32p),
+                ("ovl_off", _i32p), ("ovl_rid1", _i32p), ("ovl_rid2", _i32p),
+                ("R", ctypes.c_int), ("range_fwd", _i32p), ("range_inv", _i32p),
+                ("B", ctypes.c_int), ("Rb", ctypes.c_int), ("corr_rid", _i32p),
+                ("spin", ctypes.c_double), ("precision", ctypes.c_int)]
+
+
+class FlowError(RuntimeError):
+    """A C-ABI call returned an error status (the reference throws ``Exception``)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__("%s (status %d)" % (message, code))
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: Optional[str] = None) -> ctypes.CDLL:
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FlowError(SP_EARG, "HIP library %s is not built; run __graft_entry__.build()" % p)
+    lib = ctypes.CDLL(p)
+    c_ctx = ctypes.c_void_p
+    lib.sp_ctx_create.argtypes = [ctypes.POINTER(SpSu2Problem), ctypes.POINTER(c_ctx)]
+    lib.sp_ctx_destroy.argtypes = [c_ctx]
+    lib.sp_set_state.argtypes = [c_ctx, ctypes.c_double, _f64p, _f64p, _f64p]
+    lib.sp_get_state.argtypes = [c_ctx, _f64p, _f64p, _f64p, _f64p]
+    lib.sp_get_flow.argtypes = [c_ctx, _f64p, _f64p, _f64p]
+    lib.sp_compute_step.argtypes = [c_ctx]
+    lib.sp_finalize_step.argtypes = [c_ctx, ctypes.c_double]
+    lib.sp_is_diverged.argtypes = [c_ctx, _i32p]
+    lib.sp_measure_correlation.argtypes = [c_ctx, _f64p, _f64p]
+    lib.sp_synchronize.argtypes = [c_ctx]
+    lib.sp_last_error.argtypes = [c_ctx]
+    lib.sp_last_error.restype = ctypes.c_char_p
+    lib.sp_profile_enable.argtypes = [c_ctx, ctypes.c_int]
+    lib.sp_profile_get.argtypes = [c_ctx, ctypes.c_int, _f64p, ctypes.POINTER(ctypes.c_long)]
+    lib.sp_ctx_footprint.argtypes = [ctypes.POINTER(SpSu2Problem), ctypes.POINTER(ctypes.c_longlong)]
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+class ProblemBuffers:
+    """Keeps the numpy arrays alive behind an ``sp_su2_problem`` struct."""
+
+    def __init__(self, prob, precision: int = SP_PRECISION_FP64):
+        keep = [np.ascontiguousarray(prob.mesh, dtype=np.float32)]
+        for a in (prob.site_rid, prob.inv_rid, prob.ovl_off, prob.ovl_rid1, prob.ovl_rid2, prob.range_fwd,
+                  prob.range_inv, prob.corr_rid):
+            keep.append(np.ascontiguousarray(a, dtype=np.int32))
+        self.keep = keep
+        self.struct = SpSu2Problem(prob.nw, _p(keep[0], _f32p), prob.L, _p(keep[1], _i32p), _p(keep[2], _i32p),
+                                   _p(keep[3], _i32p), _p(keep[4], _i32p), _p(keep[5], _i32p), len(keep[6]),
+                                   _p(keep[6], _i32p), _p(keep[7], _i32p), prob.B, prob.Rb, _p(keep[8], _i32p),
+                                   float(prob.spin), precision)
+
+
+def footprint_bytes(prob) -> int:
+    lib = load_library()
+    buf = ProblemBuffers(prob)
+    out = ctypes.c_longlong(0)
+    rc = lib.sp_ctx_footprint(ctypes.byref(buf.struct), ctypes.byref(out))
+    if rc != SP_OK:
+        raise FlowError(rc, "sp_ctx_footprint failed")
+    return int(out.value)
+
+
+class FlowContext:
+    """One resident flow on the GPU (owner of an ``sp_ctx``)."""
+
+    def __init__(self, prob, precision: int = SP_PRECISION_FP64):
+        self.lib = load_library()
+        self.prob = prob
+        self._buf = ProblemBuffers(prob, precision)
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.sp_ctx_create(ctypes.byref(self._buf.struct), ctypes.byref(self.ctx))
+        if rc != SP_OK:
+            msg = self.lib.sp_last_error(self.ctx).decode() if self.ctx else "sp_ctx_create failed"
+            if self.ctx:
+                self.lib.sp_ctx_destroy(self.ctx)
+                self.ctx = ctypes.c_void_p()
+            raise FlowError(rc, msg)
+
+    def _check(self, rc: int, what: str):
+        if rc != SP_OK:
+            raise FlowError(rc, "%s: %s" % (what, self.lib.sp_last_error(self.ctx).decode()))
+
+    def close(self):
+        if self.ctx:
+            self.lib.sp_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, cutoff: float, v2, v4s, v4d):
+        v2, v4s, v4d = (np.ascontiguousarray(x, dtype=np.float64) for x in (v2, v4s, v4d))
+        n = self.prob.vertex_size
+        if v2.shape != (self.prob.nw,) or v4s.shape != (n,) or v4d.shape != (n,):
+            raise FlowError(SP_EARG, "state arrays have the wrong size")
+        self._check(self.lib.sp_set_state(self.ctx, float(cutoff), _p(v2, _f64p), _p(v4s, _f64p), _p(v4d, _f64p)),
+                    "sp_set_state")
+
+    def get_state(self) -> Tuple[float, np.ndarray, np.ndarray, np.ndarray]:
+        c = ctypes.c_double(0)
+        v2 = np.zeros(self.prob.nw)
+        v4s = np.zeros(self.prob.vertex_size)
+        v4d = np.zeros(self.prob.vertex_size)
+        self._check(self.lib.sp_get_state(self.ctx, ctypes.byref(c), _p(v2, _f64p), _p(v4s, _f64p),
+                                          _p(v4d, _f64p)), "sp_get_state")
+        return c.value, v2, v4s, v4d
+
+    def get_flow(self):
+        dv2 = np.zeros(self.prob.nw)
+        dss = np.zeros(self.prob.vertex_size)
+        ddd = np.zeros(self.prob.vertex_size)
+        self._check(self.lib.sp_get_flow(self.ctx, _p(dv2, _f64p), _p(dss, _f64p), _p(ddd, _f64p)), "sp_get_flow")
+        return dv2, dss, ddd
+
+    def compute_step(self):
+        self._check(self.lib.sp_compute_step(self.ctx), "sp_compute_step")
+
+    def finalize_step(self, new_cutoff: float):
+        self._check(self.lib.sp_finalize_step(self.ctx, float(new_cutoff)), "sp_finalize_step")
+
+    def is_diverged(self) -> bool:
+        d = ctypes.c_int(0)
+        self._check(self.lib.sp_is_diverged(self.ctx, ctypes.byref(d)), "sp_is_diverged")
+        return bool(d.value)
+
+    def measure_correlation(self):
+        n = self.prob.B * self.prob.Rb
+        zz = np.zeros(n)
+        dd = np.zeros(n)
+        self._check(self.lib.sp_measure_correlation(self.ctx, _p(zz, _f64p), _p(dd, _f64p)), "sp_measure_correlation")
+        return zz.reshape(self.prob.B, self.prob.Rb), dd.reshape(self.prob.B, self.prob.Rb)
+
+    def synchronize(self):
+        self._check(self.lib.sp_synchronize(self.ctx), "sp_synchronize")
+
+    def profile(self, enable: bool):
+        self._check(self.lib.sp_profile_enable(self.ctx, 1 if enable else 0), "sp_profile_enable")
+
+    def profile_get(self, kernel: int) -> Tuple[float, int]:
+        ms = ctypes.c_double(0)
+        n = ctypes.c_long(0)
+        self._check(self.lib.sp_profile_get(self.ctx, kernel, ctypes.byref(ms), ctypes.byref(n)), "sp_profile_get")
+        return ms.value, n.value

@@ -1,0 +1,210 @@
+"""Python mirror of the reference's FrgCore plugin boundary over the C ABI.
+
+Names and argument meaning follow the reference so that a flow is driven the
+same way ``SpinParser::runCore`` drives it:
+
+* ``EffectiveAction`` (``src/EffectiveAction.hpp:19-60``): ``cutoff``,
+  ``isDiverged()``;
+* ``FrgCore`` (``src/FrgCore.hpp:28-141``): ``computeStep()``,
+  ``finalizeStep(newCutoff)``, ``takeMeasurements()``, ``flowingFunctional()``,
+  ``flow()``;
+* ``FrgCoreFactory.newFrgCore(identifier, ...)`` (``src/FrgCoreFactory.cpp:25-52``)
+  selects the GPU core for ``symmetry="SU2"`` when the model carries the option
+  ``<backend>hip</backend>`` (or when ``backend="hip"`` is passed);
+* ``run_core`` restates the cutoff loop of ``src/SpinParser.cpp:126-186``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .capi import FlowContext, SP_PRECISION_FP64
+from .problem import SU2Problem, su2_problem
+from .task import MeasurementSpec, Task
+
+
+class InitializationError(RuntimeError):
+    """``Exception::Type::InitializationError``."""
+
+
+class ArgumentError(ValueError):
+    """``Exception::Type::ArgumentError``."""
+
+
+class EffectiveAction:
+    """State view of a GPU-resident flow (``src/EffectiveAction.hpp:19-60``)."""
+
+    def __init__(self, ctx: FlowContext, which: str):
+        self._ctx = ctx
+        self._which = which  # "state" or "flow"
+        self._cutoff = 0.0
+
+    @property
+    def cutoff(self) -> float:
+        return self._ctx_cutoff() if self._which == "state" else self._cutoff
+
+    def _ctx_cutoff(self) -> float:
+        return self._core_cutoff
+
+    def isDiverged(self) -> bool:
+        if self._which != "flow":
+            raise ArgumentError("isDiverged is evaluated on the flow (FrgCore::_flow)")
+        return self._ctx.is_diverged()
+
+    def arrays(self):
+        """Download (v2, v4ss, v4dd) in the reference layout."""
+        if self._which == "state":
+            _, v2, v4s, v4d = self._ctx.get_state()
+            return v2, v4s, v4d
+        return self._ctx.get_flow()
+
+
+class Measurement:
+    """``src/Measurement.hpp``: cutoff window and defer flag."""
+
+    def __init__(self, spec: MeasurementSpec):
+        self.spec = spec
+
+    def minCutoff(self) -> float:
+        return self.spec.min_cutoff
+
+    def maxCutoff(self) -> float:
+        return self.spec.max_cutoff
+
+    def isDeferred(self) -> bool:
+        return self.spec.defer
+
+
+@dataclass
+class CorrelationRecord:
+    cutoff: float
+    zz: np.ndarray   # SU2CorZZ data, (basis, sites in range)
+    dd: np.ndarray   # SU2CorDD data
+
+
+class SU2MeasurementCorrelation(Measurement):
+    """GPU correlation measurement (``src/SU2/SU2MeasurementCorrelation.cpp:66-177``).
+    Records are kept in memory in the order the reference writes ``measurement_k``
+    groups; duplicates at the same cutoff are discarded like the reference does
+    (``:318-322``)."""
+
+    def __init__(self, spec: MeasurementSpec):
+        super().__init__(spec)
+        self.records: List[CorrelationRecord] = []
+
+    def takeMeasurement(self, core: "HipSU2FrgCore", isMasterTask: bool = True) -> None:
+        cutoff = core.cutoff
+        if self.records and np.float32(self.records[-1].cutoff) == np.float32(cutoff):
+            return
+        zz, dd = core.context.measure_correlation()
+        if isMasterTask:
+            self.records.append(CorrelationRecord(cutoff, zz, dd))
+
+
+class FrgCore:
+    """Abstract core (``src/FrgCore.hpp:28-141``)."""
+
+    def __init__(self, measurements: List[Measurement]):
+        self._measurements = list(measurements)
+
+    def computeStep(self) -> None:
+        raise NotImplementedError
+
+    def finalizeStep(self, newCutoff: float) -> None:
+        raise NotImplementedError
+
+    def measurements(self) -> List[Measurement]:
+        return self._measurements
+
+    def takeMeasurements(self) -> None:
+        # non-deferred measurements inside their cutoff window (src/FrgCore.hpp:49-58)
+        for m in self._measurements:
+            if m.minCutoff() <= self.cutoff <= m.maxCutoff() and not m.isDeferred():
+                m.takeMeasurement(self)
+
+
+class HipSU2FrgCore(FrgCore):
+    """SU(2) core on the GPU: state, flow and tables resident in HBM."""
+
+    def __init__(self, problem: SU2Problem, measurements: List[Measurement], precision: int = SP_PRECISION_FP64):
+        super().__init__(measurements)
+        self.problem = problem
+        self.spinLength = float(problem.spin)
+        self.normalization = float(problem.normalization)
+        self.context = FlowContext(problem, precision)
+        self.cutoff = float(problem.cutoffs[0])
+        v2, v4s, v4d = problem.initial_state(np.float64)
+        self.context.set_state(self.cutoff, v2, v4s, v4d)
+        self._state = EffectiveAction(self.context, "state")
+        self._state._core_cutoff = self.cutoff
+        self._flow = EffectiveAction(self.context, "flow")
+
+    def computeStep(self) -> None:
+        self.context.compute_step()
+        self._flow._cutoff = self.cutoff
+
+    def finalizeStep(self, newCutoff: float) -> None:
+        self.context.finalize_step(float(newCutoff))
+        self.cutoff = float(newCutoff)
+        self._state._core_cutoff = self.cutoff
+
+    def flowingFunctional(self) -> EffectiveAction:
+        return self._state
+
+    def flow(self) -> EffectiveAction:
+        return self._flow
+
+    def setState(self, cutoff: float, v2, v4s, v4d) -> None:
+        """``readCheckpoint`` equivalent: replace the resident state."""
+        self.context.set_state(cutoff, v2, v4s, v4d)
+        self.cutoff = float(cutoff)
+        self._state._core_cutoff = self.cutoff
+
+    def close(self) -> None:
+        self.context.close()
+
+
+class FrgCoreFactory:
+    @staticmethod
+    def newFrgCore(identifier: str, problem: SU2Problem, measurements: List[MeasurementSpec],
+                   options: Optional[Dict[str, str]] = None, backend: Optional[str] = None) -> FrgCore:
+        options = dict(options or {})
+        backend = backend or options.pop("backend", None)
+        objs: List[Measurement] = []
+        for spec in measurements:
+            if spec.identifier != "correlation":
+                raise InitializationError("Measurement: Unknown measurement type '%s'." % spec.identifier)
+            if identifier != "SU2":
+                raise InitializationError("Measurement [correlation]: no GPU measurement for '%s'." % identifier)
+            objs.append(SU2MeasurementCorrelation(spec))
+        if identifier == "SU2" and backend == "hip":
+            return HipSU2FrgCore(problem, objs)
+        if identifier in ("SU2", "XYZ", "TRI"):
+            raise ArgumentError("Core '%s' with backend '%s' is not provided by this engine" % (identifier, backend))
+        raise ArgumentError("Spin model identifier '%s' does not exist." % identifier)
+
+
+def new_core_from_task(task: Task, resource_path: Optional[str] = None, backend: str = "hip") -> HipSU2FrgCore:
+    prob = su2_problem(task, resource_path)
+    options = {k: v for k, v in task.model_options.items() if k in ("spin", "normalization", "backend")}
+    return FrgCoreFactory.newFrgCore(task.symmetry, prob, task.measurements or [MeasurementSpec("correlation")],
+                                     options, backend=backend)
+
+
+def run_core(core: FrgCore, cutoffs: np.ndarray) -> bool:
+    """The cutoff loop of ``SpinParser::runCore`` (``src/SpinParser.cpp:141-172``).
+    Returns False if the flow diverged."""
+    diverged = False
+    k = 0
+    while k < len(cutoffs) - 1:
+        core.computeStep()
+        core.takeMeasurements()
+        if core.flow().isDiverged():
+            diverged = True
+            break
+        k += 1
+        core.finalizeStep(float(cutoffs[k]))
+    core.takeMeasurements()
+    return not diverged
