@@ -34,6 +34,8 @@ extern "C" {
 #define SP_ESTATE 4    /* call out of order (e.g. flow read before a step)     */
 
 #define SP_PRECISION_FP64 0   /* plain FP64 form                              */
+#define SP_PRECISION_MIXED 1  /* FP32 frequency integrals with compensated sums;
+                                 FP64 state, Euler step, self-energy, measurement */
 
 /* Problem descriptor: the flattened lattice / mesh tables of one SU(2) flow.
  * Built host-side by the lattice factory; replaces what the reference reads
@@ -55,7 +57,7 @@ typedef struct sp_su2_problem {
   int Rb;                 /* sites in range of each basis site                 */
   const int *corr_rid;    /* B*Rb: symmetryTransform(b, j) for the .obs output */
   double spin;            /* spin length S (core option "spin")                */
-  int precision;          /* SP_PRECISION_FP64                                 */
+  int precision;          /* SP_PRECISION_FP64 or SP_PRECISION_MIXED           */
 } sp_su2_problem;
 
 typedef struct sp_ctx sp_ctx;
@@ -119,6 +121,24 @@ int sp_profile_get(sp_ctx *ctx, int kernel, double *total_ms, long *launches);
 
 /* Device bytes held by the context (admission by footprint). */
 int sp_ctx_footprint(const sp_su2_problem *problem, long long *bytes);
+
+/* Flow groups (no reference counterpart; replaces running several SpinParser
+ * processes side by side).  A group batches resident flows of any sizes into
+ * one launch per kernel and step.  sp_group_step performs, for every flow i,
+ * one iteration of the reference's cutoff loop (src/SpinParser.cpp:141-160):
+ * computeStep at the flow's current cutoff, the correlation measurement
+ * (zz[i], dd[i], B*Rb values each; null skips), the divergence check
+ * (diverged[i]) and finalizeStep to new_cutoffs[i]; a non-finite or
+ * non-positive new_cutoffs[i], or a diverged flow, keeps its state.  The call
+ * is synchronous; contexts must not have work queued on their own streams.
+ * sp_group_measure performs the measurement only (the final one of a flow). */
+typedef struct sp_group sp_group;
+int sp_group_create(int capacity, sp_group **out);
+int sp_group_destroy(sp_group *group);
+int sp_group_step(sp_group *group, sp_ctx *const *ctxs, int n, const double *new_cutoffs, double *const *zz,
+                  double *const *dd, int *diverged);
+int sp_group_measure(sp_group *group, sp_ctx *const *ctxs, int n, double *const *zz, double *const *dd);
+const char *sp_group_last_error(const sp_group *group);
 
 #ifdef __cplusplus
 }

@@ -19,13 +19,15 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 
 SP_OK, SP_EARG, SP_ENOMEM, SP_EHIP, SP_ESTATE = 0, 1, 2, 3, 4
 SP_PRECISION_FP64 = 0
+SP_PRECISION_MIXED = 1
 
 KERNEL_DGAMMA, KERNEL_DSIGMA, KERNEL_WEIGHTS, KERNEL_EULER = 0, 1, 2, 3
 
 EXPORTED_SYMBOLS = (
     "sp_ctx_create", "sp_ctx_destroy", "sp_set_state", "sp_get_state", "sp_get_flow", "sp_compute_step",
     "sp_finalize_step", "sp_is_diverged", "sp_measure_correlation", "sp_synchronize", "sp_last_error",
-    "sp_profile_enable", "sp_profile_get", "sp_ctx_footprint",
+    "sp_profile_enable", "sp_profile_get", "sp_ctx_footprint", "sp_group_create", "sp_group_destroy",
+    "sp_group_step", "sp_group_measure", "sp_group_last_error",
 )
 
 
@@ -73,6 +75,14 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.sp_profile_enable.argtypes = [c_ctx, ctypes.c_int]
     lib.sp_profile_get.argtypes = [c_ctx, ctypes.c_int, _f64p, ctypes.POINTER(ctypes.c_long)]
     lib.sp_ctx_footprint.argtypes = [ctypes.POINTER(SpSu2Problem), ctypes.POINTER(ctypes.c_longlong)]
+    c_grp = ctypes.c_void_p
+    _pp = ctypes.POINTER(ctypes.c_void_p)
+    lib.sp_group_create.argtypes = [ctypes.c_int, ctypes.POINTER(c_grp)]
+    lib.sp_group_destroy.argtypes = [c_grp]
+    lib.sp_group_step.argtypes = [c_grp, _pp, ctypes.c_int, _f64p, ctypes.POINTER(_f64p), ctypes.POINTER(_f64p), _i32p]
+    lib.sp_group_measure.argtypes = [c_grp, _pp, ctypes.c_int, ctypes.POINTER(_f64p), ctypes.POINTER(_f64p)]
+    lib.sp_group_last_error.argtypes = [c_grp]
+    lib.sp_group_last_error.restype = ctypes.c_char_p
     if path is None:
         _lib = lib
     return lib
@@ -97,9 +107,9 @@ class ProblemBuffers:
                                    float(prob.spin), precision)
 
 
-def footprint_bytes(prob) -> int:
+def footprint_bytes(prob, precision: int = SP_PRECISION_FP64) -> int:
     lib = load_library()
-    buf = ProblemBuffers(prob)
+    buf = ProblemBuffers(prob, precision)
     out = ctypes.c_longlong(0)
     rc = lib.sp_ctx_footprint(ctypes.byref(buf.struct), ctypes.byref(out))
     if rc != SP_OK:
@@ -110,8 +120,8 @@ def footprint_bytes(prob) -> int:
 class FlowContext:
     """One resident flow on the GPU (owner of an ``sp_ctx``)."""
 
-    def __init__(self, prob, precision: int = SP_PRECISION_FP64):
-        self.lib = load_library()
+    def __init__(self, prob, precision: int = SP_PRECISION_FP64, lib: Optional[ctypes.CDLL] = None):
+        self.lib = lib or load_library()
         self.prob = prob
         self._buf = ProblemBuffers(prob, precision)
         self.ctx = ctypes.c_void_p()
@@ -191,3 +201,53 @@ class FlowContext:
         n = ctypes.c_long(0)
         self._check(self.lib.sp_profile_get(self.ctx, kernel, ctypes.byref(ms), ctypes.byref(n)), "sp_profile_get")
         return ms.value, n.value
+
+
+class FlowGroup:
+    """Batch of resident flows advanced by one launch per kernel and step
+    (``sp_group_*``)."""
+
+    def __init__(self, capacity: int):
+        self.lib = load_library()
+        self.grp = ctypes.c_void_p()
+        rc = self.lib.sp_group_create(int(capacity), ctypes.byref(self.grp))
+        if rc != SP_OK:
+            raise FlowError(rc, "sp_group_create failed")
+        self.capacity = capacity
+
+    def close(self):
+        if self.grp:
+            self.lib.sp_group_destroy(self.grp)
+            self.grp = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != SP_OK:
+            raise FlowError(rc, "%s: %s" % (what, self.lib.sp_group_last_error(self.grp).decode()))
+
+    def step(self, flows, new_cutoffs):
+        """One cutoff-loop iteration for every flow; returns ([(zz, dd)], diverged[])."""
+        n = len(flows)
+        ctxs = (ctypes.c_void_p * n)(*[f.ctx.value for f in flows])
+        nc = np.ascontiguousarray(new_cutoffs, dtype=np.float64)
+        outs = [(np.zeros(f.prob.B * f.prob.Rb), np.zeros(f.prob.B * f.prob.Rb)) for f in flows]
+        zz = (_f64p * n)(*[_p(o[0], _f64p) for o in outs])
+        dd = (_f64p * n)(*[_p(o[1], _f64p) for o in outs])
+        div = np.zeros(n, dtype=np.int32)
+        self._check(self.lib.sp_group_step(self.grp, ctxs, n, _p(nc, _f64p), zz, dd, _p(div, _i32p)), "sp_group_step")
+        return [(o[0].reshape(f.prob.B, f.prob.Rb), o[1].reshape(f.prob.B, f.prob.Rb)) for o, f in zip(outs, flows)], \
+            div.astype(bool)
+
+    def measure(self, flows):
+        n = len(flows)
+        ctxs = (ctypes.c_void_p * n)(*[f.ctx.value for f in flows])
+        outs = [(np.zeros(f.prob.B * f.prob.Rb), np.zeros(f.prob.B * f.prob.Rb)) for f in flows]
+        zz = (_f64p * n)(*[_p(o[0], _f64p) for o in outs])
+        dd = (_f64p * n)(*[_p(o[1], _f64p) for o in outs])
+        self._check(self.lib.sp_group_measure(self.grp, ctxs, n, zz, dd), "sp_group_measure")
+        return [(o[0].reshape(f.prob.B, f.prob.Rb), o[1].reshape(f.prob.B, f.prob.Rb)) for o, f in zip(outs, flows)]

@@ -1,9 +1,11 @@
-// MI355X (gfx950) kernels and C ABI of the SU(2) pf-FRG flow engine, FP64 form.
+// MI355X (gfx950) kernels and C ABI of the SU(2) pf-FRG flow engine.
 //
-// Device data layout
+// Device data layout (per flow)
 //   G, dG   vertex and its flow, interleaved channels: double2 {spin, density}
 //           at [(tri(s,u) * Nw + t) * L + rid], tri(s,u) = s(s+1)/2 + u, s >= u.
 //           One 16-byte load fetches both channels of a support point.
+//   G32     FP32 shadow of G (float2, same index) for the mixed-precision form;
+//           rewritten by every Euler step from the FP64 master copy.
 //   v2,dv2  self-energy and its flow, Nw doubles.
 //   W       per exact-frequency index x: the evaluation points w' of the
 //           frequency integral of a channel whose transfer frequency is
@@ -11,20 +13,31 @@
 //           windows of the Katanin term) with their coefficients
 //           c = trapezoid weight * propagator product.  Depends only on
 //           (x, Lambda), so it is built once per step for all items.
+//   FlowDev per-flow descriptor (pointers, sizes, cutoff, Euler step).  Every
+//           kernel takes an array of them, so one launch serves one flow or a
+//           batch of resident flows of different sizes (the served sweep).
 //
-// Kernels (one derivative evaluation = k_weights, k_dsigma, k_weights, k_dgamma):
-//   k_dsigma   one wavefront per mesh point; lanes split the lattice range sum.
-//   k_weights  one thread per exact index: window points and coefficients.
-//   k_dgamma   one 256-thread workgroup per frequency triple (s,t,u); threads
-//              are (slot, rid) pairs, slots share the evaluation points of a
-//              batch.  Per batch: (A) access-buffer descriptors (bracket search
-//              on the LDS mesh, sign / site-exchange folding) one per thread,
-//              (B) 16-byte vertex gathers and the channel algebra per (slot,
-//              rid), (C, t channel only) the RPA lattice bubble accumulated as
-//              C[a][b] += sum_e c_e A2_e[a] A3_e[b] in LDS, contracted with the
-//              overlap lists once per item.
-//   k_euler    in-place Gamma += dLambda * dGamma (streaming, HBM bound).
-//   k_corr_*   static susceptibility, one workgroup per outer frequency.
+// Kernels (one derivative evaluation = k_dsigma, k_weights, k_dgamma):
+//   k_dsigma   one wavefront per (flow, mesh point); lanes split the range sum.
+//   k_weights  one thread per (flow, exact index): window points, coefficients.
+//   k_dgamma   one 256-thread workgroup per (flow, frequency triple (s,t,u)),
+//              blocks mapped to XCDs in contiguous item ranges.  Threads are
+//              (slot, rid) pairs; per batch of evaluation points:
+//              (A) one thread per (point, pair of access buffers): the two
+//                  bracket searches a pair shares, sign / site-exchange folding,
+//                  support byte offsets and weights -> LDS;
+//              (B) per (slot, rid): 4 x 4 vertex gathers (buffer loads with
+//                  32-bit offsets) and the channel algebra;
+//              (C, t channel) the RPA lattice bubble accumulated as
+//                  C[a][b] += sum_e c_e A2_e[a] A3_e[b] in LDS (2x2 register
+//                  tiles), contracted with the overlap lists once per item.
+//              Template MIXED: FP32 gathers and algebra with compensated
+//              (two-sum) accumulation; descriptors, frequencies and the result
+//              stay FP64.
+//   k_euler    in-place Gamma += dLambda * dGamma (streaming, HBM bound);
+//              skipped for a flow whose derivative holds a NaN.
+//   k_corr_*   static susceptibility in FP64, one workgroup per (flow, outer
+//              frequency).
 //
 // Reference semantics followed (paths relative to the SpinParser repository):
 //   src/FrequencyDiscretization.hpp:253-356, src/lib/Integrator.hpp:137-287,
@@ -35,21 +48,43 @@
 // -ffp-contract=off); only the accumulations use explicit fma().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "spfrg.h"
 
 namespace {
 
-constexpr int kThreads = 256;
+// launch-shape tunables of k_dgamma (compile-time; the defaults are the tuned ones)
+#ifndef SPFRG_THREADS
+#define SPFRG_THREADS 256
+#endif
+#ifndef SPFRG_MAX_BATCH
+#define SPFRG_MAX_BATCH 32
+#endif
+#ifndef SPFRG_MIN_BLOCKS
+#define SPFRG_MIN_BLOCKS 4
+#endif
+constexpr int kThreads = SPFRG_THREADS;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoPi = 2.0 * kPi;
-constexpr int kMaxBatch = 32;
+constexpr int kMaxBatch = SPFRG_MAX_BATCH;
+#ifndef SPFRG_UNROLL_FP64
+#define SPFRG_UNROLL_FP64 1
+#endif
+#ifndef SPFRG_UNROLL_MIXED
+#define SPFRG_UNROLL_MIXED 1
+#endif
+#ifndef SPFRG_CHUNK
+#define SPFRG_CHUNK 16
+#endif
+constexpr long long kChunk = SPFRG_CHUNK;
 
 // ---------------------------------------------------------------------------
 // frequency mesh (positive half m[0..n) ascending; signed index i in [-n, n))
@@ -233,23 +268,59 @@ __device__ void katanin_windows(const double *m, int n, double X, double cut, Po
   if (cut < mesh_at(m, n - 1)) window_left(m, n, cut, o);
 }
 
+// ---------------------------------------------------------------------------
+// per-flow descriptor
+// ---------------------------------------------------------------------------
+struct FlowDev {
+  double2 *G;
+  double2 *dG;
+  float2 *G32;  // FP32 shadow of G (mixed precision), null otherwise
+  double *v2, *dv2;
+  const double *mesh;
+  const int *site_rid, *inv_rid, *ovl_off, *ovl_rid1, *ovl_rid2, *range_fwd, *range_inv, *corr_rid;
+  double *Wwp, *Wc;
+  int *Wn;
+  double *Cwp, *Cw;
+  int *Cn;
+  double *corr_partial, *corr_out;
+  int *flags;  // [0] NaN in the flow, [1] evaluation-list overflow
+  int nw, L, R, B, Rb, emax, nb, slots, precision, euler;
+  long long nf, item_base, nv;
+  unsigned g_bytes, g32_bytes;
+  double spin, cut, step;
+  int pad_index;  // host bookkeeping: position of the flow in the caller's array
+};
+
+__device__ __forceinline__ int flow_of(const FlowDev *__restrict__ fl, int nflows, long long item) {
+  int lo = 0, hi = nflows - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fl[mid].item_base <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // W table: x < nw -> channel lists with propagators; x == nw -> plain
 // correlation list.  flags[1] is raised if a list overflows its capacity.
-__global__ void k_weights(const double *__restrict__ m, int nw, const double *__restrict__ v2,
-                          const double *__restrict__ dv2, double cut, int emax, double *Wwp, double *Wc, int *Wn,
-                          double *Cwp, double *Cw, int *Cn, int corr_only, int *flags) {
+__global__ void k_weights(const FlowDev *__restrict__ fl, int corr_only) {
+  const FlowDev &F = fl[blockIdx.y];
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nw = F.nw, emax = F.emax;
+  const double *m = F.mesh;
+  const double cut = F.cut;
   if (x > nw) return;
   if (x == nw) {
-    PointList o{Cwp, Cw, 0, emax};
+    PointList o{F.Cwp, F.Cw, 0, emax};
     katanin_windows(m, nw, 0.0, cut, o);
-    *Cn = o.cnt < emax ? o.cnt : emax;
-    if (o.cnt > emax) atomicOr(&flags[1], 1);
+    *F.Cn = o.cnt < emax ? o.cnt : emax;
+    if (o.cnt > emax) atomicOr(&F.flags[1], 1);
     return;
   }
   if (corr_only) return;
+  const double *v2 = F.v2, *dv2 = F.dv2;
   const double X = m[x];
-  PointList o{Wwp + (size_t)x * emax, Wc + (size_t)x * emax, 0, emax};
+  PointList o{F.Wwp + (size_t)x * emax, F.Wc + (size_t)x * emax, 0, emax};
   // conventional contribution (src/SU2/SU2FrgCore.cpp:349-371): p(L, L +- X)
   auto p = [&](double a, double b) { return 1.0 / ((a + sigma_at(m, nw, v2, a)) * (b + sigma_at(m, nw, v2, b))); };
   o.push(cut, p(cut, cut + X));
@@ -263,206 +334,210 @@ __global__ void k_weights(const double *__restrict__ m, int nw, const double *__
     const double d1 = wp + sigma_at(m, nw, v2, wp);
     o.w[k] *= sigma_at(m, nw, dv2, wp) / (d1 * d1 * (X + wp + sigma_at(m, nw, v2, X + wp)));
   }
-  Wn[x] = kend;
-  if (o.cnt > emax) atomicOr(&flags[1], 1);
+  F.Wn[x] = kend;
+  if (o.cnt > emax) atomicOr(&F.flags[1], 1);
 }
 
 // ---------------------------------------------------------------------------
 // dSigma/dLambda (src/SU2/SU2FrgCore.cpp:139-169): one wavefront per mesh point
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_dsigma(const double2 *__restrict__ G, const double *__restrict__ m,
-                                               const double *__restrict__ v2, const int *__restrict__ rfwd,
-                                               const int *__restrict__ rinv, int R, int nw, int L, double cut,
-                                               double spin, double *dv2, int *flags) {
+__global__ __launch_bounds__(64) void k_dsigma(const FlowDev *__restrict__ fl) {
+  const FlowDev &F = fl[blockIdx.y];
   const int i = blockIdx.x;
+  if (i >= F.nw) return;
   const int lane = threadIdx.x;
+  const int nw = F.nw, L = F.L;
+  const double *m = F.mesh;
+  const double2 *G = F.G;
+  const double cut = F.cut;
   const double w = m[i];
   double sum = 0.0;
-  for (int j = lane; j < R; j += 64) {
-    const double2 a = value_none(G, m, nw, L, rfwd[j], rinv[j], w + cut, 0.0, w - cut);
-    const double2 b = value_none(G, m, nw, L, rfwd[j], rinv[j], w - cut, 0.0, w + cut);
+  for (int j = lane; j < F.R; j += 64) {
+    const double2 a = value_none(G, m, nw, L, F.range_fwd[j], F.range_inv[j], w + cut, 0.0, w - cut);
+    const double2 b = value_none(G, m, nw, L, F.range_fwd[j], F.range_inv[j], w - cut, 0.0, w + cut);
     sum += a.y - b.y;
   }
   sum = wave_sum(sum);
   if (lane == 0) {
     double v = 0.0;
-    v -= 4.0 * spin * sum;
+    v -= 4.0 * F.spin * sum;
     const double2 p = value_none(G, m, nw, L, 0, 0, w + cut, w - cut, 0.0);
     const double2 q = value_none(G, m, nw, L, 0, 0, w - cut, w + cut, 0.0);
     v += 0.75 * (p.x - q.x);
     v += (p.y - q.y);
-    v /= (kTwoPi * (cut + sigma_at(m, nw, v2, cut)));
-    dv2[i] = v;
-    if (isnan(v)) atomicOr(&flags[0], 1);
+    v /= (kTwoPi * (cut + sigma_at(m, nw, F.v2, cut)));
+    F.dv2[i] = v;
+    if (isnan(v)) atomicOr(&F.flags[0], 1);
   }
 }
 
 // ---------------------------------------------------------------------------
 // dGamma/dLambda
 // ---------------------------------------------------------------------------
-struct GammaArgs {
-  const double2 *__restrict__ G;
-  double2 *__restrict__ dG;
-  const double *__restrict__ mesh;
-  const int *__restrict__ site_rid;
-  const int *__restrict__ inv_rid;
-  const int *__restrict__ ovl_off;
-  const int *__restrict__ ovl_rid1;
-  const int *__restrict__ ovl_rid2;
-  const double *__restrict__ Wwp;
-  const double *__restrict__ Wc;
-  const int *__restrict__ Wn;
-  int *flags;
-  int nw, L, emax, nb, slots, item0;
-  double two_s, eight_s;
-};
-
-// LDS carve-up (bytes), computed identically on host and device
+// LDS carve-up (bytes), computed identically on host and device.  Lp = L
+// rounded up to even (2x2 register tiles of the lattice-bubble update); vs =
+// size of the gather type (8 FP64, 4 mixed).
 struct LdsLayout {
-  int mesh, wp, c, w, loc, P, Q, C, rpa, red, row, flag, site, inv, total;
+  int mesh, c, off, ws, wd, ex, loc, P, Q, C, rpa, red, site, inv, total;
 };
 
-__host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots) {
+__host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots, int vs) {
   LdsLayout l;
+  const int Lp = L + (L & 1);
   int off = 0;
   auto take = [&](int bytes) { int o = off; off += (bytes + 15) & ~15; return o; };
   l.mesh = take(nw * 8);
-  l.wp = take(nb * 8);
   l.c = take(nb * 8);
-  l.w = take(nb * 4 * 4 * 8);
-  l.loc = take(nb * 4 * 2 * 8);
-  const int pq = 2 * nb * L * 8;
+  l.off = take(nb * 4 * 16);
+  l.ws = take(nb * 4 * 4 * vs);
+  l.wd = take(nb * 4 * 4 * vs);
+  l.ex = take(nb * 4 * 4);
+  l.loc = take(nb * 4 * 2 * vs);
+  const int pq = 2 * nb * Lp * vs;
   const int red = slots * L * 2 * 8;
   l.P = take(pq > red ? pq : red);  // the final slot reduction reuses P
   l.red = l.P;
   l.Q = take(pq);
-  l.C = take(2 * L * L * 8);
+  l.C = take(2 * Lp * Lp * 8);
   l.rpa = take(2 * L * 8);
-  l.row = take(nb * 4 * 4 * 4);
-  l.flag = take(nb * 4 * 4);
   l.site = take(L * 4);
   l.inv = take(L * 4);
   l.total = off;
   return l;
 }
 
-// access-buffer descriptor: 4 support rows, weights, density-sign mask, exchange
-struct Desc {
-  int row[4];
-  double w[4];
-  int flags;  // bit i: negate density of support i; bit 4: site exchange
+struct Interp {
+  int lo, hi;
+  double b;
 };
 
-// SU2VertexTwoParticle::generateAccessBuffer(s, t, u, channel) (:399-490) with
-// the exact argument's mesh index given: chan 0 = s exact, 1 = t, 2 = u
-__device__ __forceinline__ Desc make_desc(const double *m, int nw, double a, double b, double c, int chan, int ex) {
-  Desc d;
-  int fl = 0;
-  if (a < 0 && c < 0) { a = -a; c = -c; }
-  else {
-    if (a < 0) { a = -a; fl |= 16; }
-    else if (c < 0) { c = -c; fl |= 16; }
-  }
-  if (b < 0) b = -b;
-  int lo1, hi1, lo2, hi2;
-  double b1, b2;
-  bool n0, n1, n2, n3;
-  if (chan == 0) {
-    interp(m, nw, b, lo1, hi1, b1);
-    interp(m, nw, c, lo2, hi2, b2);
-    d.row[0] = support_row(ex, lo1, lo2, nw, n0);
-    d.row[1] = support_row(ex, hi1, lo2, nw, n1);
-    d.row[2] = support_row(ex, lo1, hi2, nw, n2);
-    d.row[3] = support_row(ex, hi1, hi2, nw, n3);
-  } else if (chan == 1) {
-    interp(m, nw, a, lo1, hi1, b1);
-    interp(m, nw, c, lo2, hi2, b2);
-    d.row[0] = support_row(lo1, ex, lo2, nw, n0);
-    d.row[1] = support_row(hi1, ex, lo2, nw, n1);
-    d.row[2] = support_row(lo1, ex, hi2, nw, n2);
-    d.row[3] = support_row(hi1, ex, hi2, nw, n3);
-  } else {
-    interp(m, nw, a, lo1, hi1, b1);
-    interp(m, nw, b, lo2, hi2, b2);
-    d.row[0] = support_row(lo1, lo2, ex, nw, n0);
-    d.row[1] = support_row(hi1, lo2, ex, nw, n1);
-    d.row[2] = support_row(lo1, hi2, ex, nw, n2);
-    d.row[3] = support_row(hi1, hi2, ex, nw, n3);
-  }
-  d.w[0] = (1 - b2) * (1 - b1);
-  d.w[1] = (1 - b2) * b1;
-  d.w[2] = b2 * (1 - b1);
-  d.w[3] = b2 * b1;
-  d.flags = fl | (n0 ? 1 : 0) | (n1 ? 2 : 0) | (n2 ? 4 : 0) | (n3 ? 8 : 0);
+__device__ __forceinline__ Interp interp_of(const double *m, int nw, double x) {
+  Interp r;
+  interp(m, nw, x, r.lo, r.hi, r.b);
+  return r;
+}
+
+template <class V2> __device__ __forceinline__ V2 buf_load(__amdgpu_buffer_rsrc_t r, int byte_off);
+
+template <> __device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
+  double2 d;
+  __builtin_memcpy(&d, &v, 16);
   return d;
 }
 
-// frequency arguments (s, t, u) of access buffer k of channel ch
-// (src/SU2/SU2FrgCore.cpp:199-334)
-__device__ __forceinline__ void buffer_args(int ch, int k, double s, double t, double u, double w1p, double w1,
-                                            double w2p, double w2, double wp, double &a, double &b, double &c,
-                                            int &chan) {
-  if (ch == 0) {
-    chan = 0;
-    a = s;
-    switch (k) {
-      case 0: b = -w1 - wp; c = -w2 - wp; break;
-      case 1: b = w1p + wp; c = -w2p - wp; break;
-      case 2: b = w2 + wp; c = w1 + wp; break;
-      default: b = -w2p - wp; c = w1p + wp; break;
-    }
-  } else if (ch == 1) {
-    if (k < 4) {
-      chan = 1;
-      b = t;
-      switch (k) {
-        case 0: a = w1 - wp; c = w1p + wp; break;
-        case 1: a = w2p - wp; c = -w2 - wp; break;
-        case 2: a = w1p + wp; c = w1 - wp; break;
-        default: a = w2 + wp; c = wp - w2p; break;
-      }
-    } else {
-      chan = 2;
-      switch (k) {
-        case 4: a = w2p - wp; b = -w2 - wp; c = t; break;
-        case 5: a = w1 - wp; b = -w1p - wp; c = -t; break;
-        case 6: a = w2 + wp; b = wp - w2p; c = t; break;
-        default: a = w1p + wp; b = wp - w1; c = -t; break;
-      }
-    }
-  } else {
-    chan = 2;
-    c = u;
-    switch (k) {
-      case 0: a = w1 + wp; b = wp - w2p; break;
-      case 1: a = w1p + wp; b = w2 - wp; break;
-      case 2: a = w2p - wp; b = -w1 - wp; break;
-      default: a = w2 - wp; b = w1p + wp; break;
-    }
-  }
+template <> __device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+  float2 d;
+  __builtin_memcpy(&d, &v, 8);
+  return d;
 }
 
-__global__ __launch_bounds__(kThreads) void k_dgamma(GammaArgs g) {
+template <class V2, class V> __device__ __forceinline__ V2 mk2(V a, V b) {
+  V2 r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+
+// compensated FP32 accumulator (Knuth two-sum per addition)
+struct KSum {
+  float s, c;
+};
+
+__device__ __forceinline__ void ksum_add(KSum &k, float x) {
+  const float t = k.s + x;
+  const float z = t - k.s;
+  k.c += (k.s - (t - z)) + (x - z);
+  k.s = t;
+}
+
+// Access-buffer descriptor of SU2VertexTwoParticle::generateAccessBuffer(s, t,
+// u, channel) (:399-490) from the two interpolations it needs: chan 0 = s exact
+// (I1 on t, I2 on u), 1 = t exact (I1 on s, I2 on u), 2 = u exact (I1 on s,
+// I2 on t).  Support byte offsets are row * rowbytes; the density weight
+// carries the sign of the s <-> u exchange (:616-632).
+struct Desc4 {
+  int off[4];
+  double ws[4];
+  double wd[4];
+};
+
+__device__ __forceinline__ Desc4 assemble(int chan, int ex, Interp i1, Interp i2, int nw, int rowbytes) {
+  Desc4 d;
+  const int l1[4] = {i1.lo, i1.hi, i1.lo, i1.hi};
+  const int l2[4] = {i2.lo, i2.lo, i2.hi, i2.hi};
+  const double b1 = i1.b, b2 = i2.b;
+  const double w[4] = {(1 - b2) * (1 - b1), (1 - b2) * b1, b2 * (1 - b1), b2 * b1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int so, to, uo;
+    if (chan == 0) { so = ex; to = l1[q]; uo = l2[q]; }
+    else if (chan == 1) { so = l1[q]; to = ex; uo = l2[q]; }
+    else { so = l1[q]; to = l2[q]; uo = ex; }
+    bool neg;
+    const int row = support_row(so, to, uo, nw, neg);
+    d.off[q] = row * rowbytes;
+    d.ws[q] = w[q];
+    d.wd[q] = neg ? -w[q] : w[q];
+  }
+  return d;
+}
+
+// one access buffer of a pair: the signed s and u arguments decide the site
+// exchange (exactly one of them negative, :405-427); swap says whether the
+// pair's interpolations (I0, I1) feed the descriptor's (slot 1, slot 2) or
+// (slot 2, slot 1)
+struct PairBuf {
+  int k;
+  double a, c;
+  bool swap;
+  int chan;
+};
+
+template <bool MIXED>
+__global__ __launch_bounds__(kThreads, SPFRG_MIN_BLOCKS) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,
+                                                     long long per_xcd) {
+  using V = typename std::conditional<MIXED, float, double>::type;
+  using V2 = typename std::conditional<MIXED, float2, double2>::type;
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int nw = g.nw, L = g.L, nb = g.nb, slots = g.slots;
-  const LdsLayout lay = lds_layout(nw, L, nb, slots);
+
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs.
+  // Hand out chunks of kChunk consecutive items (one (s,u) row of t values
+  // shares the s and u vertex slabs) to the XCDs in turn, so each chunk stays
+  // in one XCD's L2 while the XCDs still receive statistically equal work.
+  const long long j = blockIdx.x >> 3;
+  const long long lb = ((j / kChunk) * 8 + (blockIdx.x & 7)) * kChunk + (j % kChunk);
+  if (lb >= total) return;
+  const int f = flow_of(fl, nflows, lb);
+  const FlowDev &F = fl[f];
+  const int item = (int)(lb - F.item_base);
+
+  const int nw = F.nw, L = F.L, nb = F.nb, slots = F.slots;
+  const int Lp = L + (L & 1);
+  const LdsLayout lay = lds_layout(nw, L, nb, slots, (int)sizeof(V));
   double *s_mesh = reinterpret_cast<double *>(lds_raw + lay.mesh);
-  double *s_wp = reinterpret_cast<double *>(lds_raw + lay.wp);
   double *s_c = reinterpret_cast<double *>(lds_raw + lay.c);
-  double *s_w = reinterpret_cast<double *>(lds_raw + lay.w);
-  double *s_loc = reinterpret_cast<double *>(lds_raw + lay.loc);
-  double *s_P = reinterpret_cast<double *>(lds_raw + lay.P);
-  double *s_Q = reinterpret_cast<double *>(lds_raw + lay.Q);
-  double *s_C = reinterpret_cast<double *>(lds_raw + lay.C);
+  int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off);
+  V2 *s_ws = reinterpret_cast<V2 *>(lds_raw + lay.ws);
+  V2 *s_wd = reinterpret_cast<V2 *>(lds_raw + lay.wd);
+  int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex);
+  V *s_loc = reinterpret_cast<V *>(lds_raw + lay.loc);
+  V *s_P = reinterpret_cast<V *>(lds_raw + lay.P);
+  V *s_Q = reinterpret_cast<V *>(lds_raw + lay.Q);
+  double *s_C = reinterpret_cast<double *>(lds_raw + lay.C);  // FP64 sums, or {sum, comp} float pairs
   double *s_rpa = reinterpret_cast<double *>(lds_raw + lay.rpa);
   double *s_red = reinterpret_cast<double *>(lds_raw + lay.red);
-  int *s_row = reinterpret_cast<int *>(lds_raw + lay.row);
-  int *s_flag = reinterpret_cast<int *>(lds_raw + lay.flag);
   int *s_site = reinterpret_cast<int *>(lds_raw + lay.site);
   int *s_inv = reinterpret_cast<int *>(lds_raw + lay.inv);
 
+  const void *gbase = MIXED ? (const void *)F.G32 : (const void *)F.G;
+  const __amdgpu_buffer_rsrc_t rG =
+      __builtin_amdgcn_make_buffer_rsrc((void *)gbase, (short)0, (int)(MIXED ? F.g32_bytes : F.g_bytes), 0x00020000);
+  const int esz = (int)sizeof(V2);
+  const int rowbytes = L * esz;
+
   const int tid = threadIdx.x;
-  const int item = blockIdx.x + g.item0;
   // SU2VertexTwoParticle::expandIterator (:136-158): item = tri(s,u) * nw + t
   const int su = item / nw;
   const int to = item - su * nw;
@@ -471,9 +546,10 @@ __global__ __launch_bounds__(kThreads) void k_dgamma(GammaArgs g) {
   while (so * (so + 1) / 2 > su) --so;
   const int uo = su - so * (so + 1) / 2;
 
-  for (int i = tid; i < nw; i += kThreads) s_mesh[i] = g.mesh[i];
-  for (int i = tid; i < L; i += kThreads) { s_site[i] = g.site_rid[i]; s_inv[i] = g.inv_rid[i]; }
-  for (int i = tid; i < 2 * L * L; i += kThreads) s_C[i] = 0.0;
+  for (int i = tid; i < nw; i += kThreads) s_mesh[i] = F.mesh[i];
+  for (int i = tid; i < L; i += kThreads) { s_site[i] = F.site_rid[i]; s_inv[i] = F.inv_rid[i]; }
+  for (int i = tid; i < 2 * Lp * Lp; i += kThreads) s_C[i] = 0.0;
+  for (int i = tid; i < 2 * nb * Lp; i += kThreads) { s_P[i] = V(0); s_Q[i] = V(0); }
   __syncthreads();
 
   const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
@@ -485,107 +561,226 @@ __global__ __launch_bounds__(kThreads) void k_dgamma(GammaArgs g) {
   const int slot = tid / L;
   const int rid = tid - slot * L;
   const bool active = slot < slots;
-  double acc_s = 0.0, acc_d = 0.0;
+  const int sr_fwd = active ? s_site[rid] * esz : 0;
+  const int sr_inv = active ? s_inv[rid] * esz : 0;
+  double acc_s = 0.0, acc_d = 0.0;   // FP64 form
+  KSum ks{0.f, 0.f}, kd{0.f, 0.f};  // mixed form
 
   for (int ch = 0; ch < 3; ++ch) {
     const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
-    const int n = g.Wn[xo];
-    const double *Wwp = g.Wwp + (size_t)xo * g.emax;
-    const double *Wc = g.Wc + (size_t)xo * g.emax;
-    const int nbuf = ch == 1 ? 8 : 4;
+    const int n = F.Wn[xo];
+    const double *Wwp = F.Wwp + (size_t)xo * F.emax;
+    const double *Wc = F.Wc + (size_t)xo * F.emax;
     const double sgn = ch == 2 ? -1.0 : 1.0;
     for (int e0 = 0; e0 < n; e0 += nb) {
       const int nvalid = n - e0 < nb ? n - e0 : nb;
-      // (A) descriptors, one (evaluation, buffer) per thread
-      if (tid < nvalid * nbuf) {
-        const int e = tid / nbuf, k = tid - e * nbuf;
+      // (A) one thread per (point, pair): the two bracket searches shared by
+      // the 2 (s, u channels) or 4 (t channel) access buffers of the pair
+      if (tid < 2 * nvalid) {
+        const int e = tid >> 1, pr = tid & 1;
         const double wp = Wwp[e0 + e];
-        double a, b, c;
-        int chan;
-        buffer_args(ch, k, s, t, u, w1p, w1, w2p, w2, wp, a, b, c, chan);
-        const Desc d = make_desc(s_mesh, nw, a, b, c, chan, xo);
-        if (k < 4) {
-          const int base = (e * 4 + k) * 4;
-          for (int i = 0; i < 4; ++i) { s_row[base + i] = d.row[i]; s_w[base + i] = d.w[i]; }
-          s_flag[e * 4 + k] = d.flags;
-        } else {
-          // SU2VertexTwoParticle::getValueLocal (:347-360): site 0, no exchange
-          double vs = 0.0, vd = 0.0;
-          for (int i = 0; i < 4; ++i) {
-            const double2 v = g.G[(size_t)d.row[i] * L];
-            vs = fma(d.w[i], v.x, vs);
-            vd = fma((d.flags >> i) & 1 ? -d.w[i] : d.w[i], v.y, vd);
+        double x0, x1;
+        PairBuf pb[4];
+        int nbufs = 2;
+        if (ch == 0) {
+          if (pr == 0) {
+            x0 = fabs(w1 + wp); x1 = fabs(w2 + wp);
+            pb[0] = {0, s, -w2 - wp, false, 0};
+            pb[1] = {2, s, w1 + wp, true, 0};
+          } else {
+            x0 = fabs(w1p + wp); x1 = fabs(w2p + wp);
+            pb[0] = {1, s, -w2p - wp, false, 0};
+            pb[1] = {3, s, w1p + wp, true, 0};
           }
-          s_loc[(e * 4 + (k - 4)) * 2 + 0] = vs;
-          s_loc[(e * 4 + (k - 4)) * 2 + 1] = vd;
+        } else if (ch == 1) {
+          nbufs = 4;
+          if (pr == 0) {
+            x0 = fabs(w1 - wp); x1 = fabs(w1p + wp);
+            pb[0] = {0, w1 - wp, w1p + wp, false, 1};
+            pb[1] = {2, w1p + wp, w1 - wp, true, 1};
+            pb[2] = {5, w1 - wp, -t, false, 2};
+            pb[3] = {7, w1p + wp, -t, true, 2};
+          } else {
+            x0 = fabs(w2p - wp); x1 = fabs(w2 + wp);
+            pb[0] = {1, w2p - wp, -w2 - wp, false, 1};
+            pb[1] = {3, w2 + wp, wp - w2p, true, 1};
+            pb[2] = {4, w2p - wp, t, false, 2};
+            pb[3] = {6, w2 + wp, t, true, 2};
+          }
+        } else {
+          if (pr == 0) {
+            x0 = fabs(w1 + wp); x1 = fabs(w2p - wp);
+            pb[0] = {0, w1 + wp, u, false, 2};
+            pb[1] = {2, w2p - wp, u, true, 2};
+          } else {
+            x0 = fabs(w1p + wp); x1 = fabs(w2 - wp);
+            pb[0] = {1, w1p + wp, u, false, 2};
+            pb[1] = {3, w2 - wp, u, true, 2};
+          }
         }
-        if (k == 0) { s_wp[e] = wp; s_c[e] = sgn * Wc[e0 + e]; }
-      }
-      __syncthreads();
-      // (B) gathers and channel algebra per (slot, rid)
-      if (active) {
-        for (int e = slot; e < nvalid; e += slots) {
-          double As[4], Ad[4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int fl = s_flag[e * 4 + k];
-            const int sr = (fl & 16) ? s_inv[rid] : s_site[rid];
-            const int base = (e * 4 + k) * 4;
-            double xs = 0.0, xd = 0.0;
+        const Interp i0 = interp_of(s_mesh, nw, x0);
+        const Interp i1 = interp_of(s_mesh, nw, x1);
+        for (int q = 0; q < nbufs; ++q) {
+          const PairBuf &b = pb[q];
+          const Desc4 d = assemble(b.chan, xo, b.swap ? i1 : i0, b.swap ? i0 : i1, nw, rowbytes);
+          if (b.k < 4) {
+            const int sk = e * 4 + b.k;
+            s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
+            s_ws[2 * sk] = mk2<V2>(V(d.ws[0]), V(d.ws[1]));
+            s_ws[2 * sk + 1] = mk2<V2>(V(d.ws[2]), V(d.ws[3]));
+            s_wd[2 * sk] = mk2<V2>(V(d.wd[0]), V(d.wd[1]));
+            s_wd[2 * sk + 1] = mk2<V2>(V(d.wd[2]), V(d.wd[3]));
+            s_ex[sk] = ((b.a < 0) != (b.c < 0)) ? 1 : 0;
+          } else {
+            // SU2VertexTwoParticle::getValueLocal (:347-360): site 0, no exchange
+            V vs = V(0), vd = V(0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              const double2 v = g.G[(size_t)s_row[base + i] * L + sr];
-              const double w = s_w[base + i];
-              xs = fma(w, v.x, xs);
-              xd = fma((fl >> i) & 1 ? -w : w, v.y, xd);
+              const V2 v = buf_load<V2>(rG, d.off[i]);
+              vs = fma(V(d.ws[i]), v.x, vs);
+              vd = fma(V(d.wd[i]), v.y, vd);
             }
-            As[k] = xs;
-            Ad[k] = xd;
+            s_loc[(e * 4 + (b.k - 4)) * 2 + 0] = vs;
+            s_loc[(e * 4 + (b.k - 4)) * 2 + 1] = vd;
           }
-          const double c = s_c[e];
-          double rs, rd;
-          if (ch == 0) {
-            // integralKernelS (src/SU2/SU2FrgCore.cpp:199-227)
-            rs = -0.5 * As[0] * As[1] - 0.5 * As[2] * As[3] + Ad[0] * As[1] + Ad[2] * As[3] + As[0] * Ad[1] +
-                 As[2] * Ad[3];
-            rd = 0.1875 * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
-          } else if (ch == 2) {
-            // integralKernelU (:305-334)
-            rs = 0.5 * As[0] * As[1] + 0.5 * As[2] * As[3] + As[0] * Ad[1] + As[2] * Ad[3] + Ad[0] * As[1] +
-                 Ad[2] * As[3];
-            rd = 0.1875 * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
-          } else {
-            // integralKernelT (:229-303) without the lattice bubble
-            const double *lc = s_loc + e * 8;
-            const double Cbs = lc[0], Cbd = lc[1], ICas = lc[2], ICad = lc[3];
-            const double Cbs2 = lc[4], Cbd2 = lc[5], ICas2 = lc[6], ICad2 = lc[7];
-            rs = (0.25 * Cbs - Cbd) * As[0] + (0.25 * ICas - ICad) * As[1] + (0.25 * Cbs2 - Cbd2) * As[2] +
-                 (0.25 * ICas2 - ICad2) * As[3];
-            rd = -(Cbd + 0.75 * Cbs) * Ad[0] - (ICad + 0.75 * ICas) * Ad[1] - (Cbd2 + 0.75 * Cbs2) * Ad[2] -
-                 (ICad2 + 0.75 * ICas2) * Ad[3];
-            // lattice-bubble factors: P = c A2, Q = A3 per channel
-            s_P[(0 * nb + e) * L + rid] = c * As[2];
-            s_P[(1 * nb + e) * L + rid] = c * Ad[2];
-            s_Q[(0 * nb + e) * L + rid] = As[3];
-            s_Q[(1 * nb + e) * L + rid] = Ad[3];
+        }
+        if (pr == 0) s_c[e] = sgn * Wc[e0 + e];
+      }
+      __syncthreads();
+      // (B) gathers and channel algebra per (slot, rid); U evaluation points
+      // per iteration so that 16 U loads are in flight before the first use
+      if (active) {
+        constexpr int U = MIXED ? SPFRG_UNROLL_MIXED : SPFRG_UNROLL_FP64;
+        for (int e = slot; e < nvalid; e += U * slots) {
+          int ev[U];
+          bool ok[U];
+          // U == 1: loads and weights per access buffer, leaving the schedule
+          // to the compiler; U > 1: all 16 U loads issued before any use
+          V2 val[U > 1 ? U : 1][16];
+#pragma unroll
+          for (int q = 0; q < U; ++q) {
+            ev[q] = e + q * slots;
+            ok[q] = ev[q] < nvalid;
+            if (!ok[q]) ev[q] = e;
+            if (U > 1) {
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                const int sk = ev[q] * 4 + k;
+                const int sr = s_ex[sk] ? sr_inv : sr_fwd;
+                const int4 o = s_off[sk];
+                val[q][4 * k + 0] = buf_load<V2>(rG, o.x + sr);
+                val[q][4 * k + 1] = buf_load<V2>(rG, o.y + sr);
+                val[q][4 * k + 2] = buf_load<V2>(rG, o.z + sr);
+                val[q][4 * k + 3] = buf_load<V2>(rG, o.w + sr);
+              }
+            }
           }
-          acc_s = fma(c, rs, acc_s);
-          acc_d = fma(c, rd, acc_d);
+#pragma unroll
+          for (int q = 0; q < U; ++q) {
+            if (!ok[q]) continue;
+            const int ee = ev[q];
+            V As[4], Ad[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const int sk = ee * 4 + k;
+              const V2 ws01 = s_ws[2 * sk], ws23 = s_ws[2 * sk + 1];
+              const V2 wd01 = s_wd[2 * sk], wd23 = s_wd[2 * sk + 1];
+              V2 v[4];
+              if (U > 1) {
+                v[0] = val[q][4 * k + 0]; v[1] = val[q][4 * k + 1];
+                v[2] = val[q][4 * k + 2]; v[3] = val[q][4 * k + 3];
+              } else {
+                const int sr = s_ex[sk] ? sr_inv : sr_fwd;
+                const int4 o = s_off[sk];
+                v[0] = buf_load<V2>(rG, o.x + sr);
+                v[1] = buf_load<V2>(rG, o.y + sr);
+                v[2] = buf_load<V2>(rG, o.z + sr);
+                v[3] = buf_load<V2>(rG, o.w + sr);
+              }
+              As[k] = fma(ws23.y, v[3].x, fma(ws23.x, v[2].x, fma(ws01.y, v[1].x, ws01.x * v[0].x)));
+              Ad[k] = fma(wd23.y, v[3].y, fma(wd23.x, v[2].y, fma(wd01.y, v[1].y, wd01.x * v[0].y)));
+            }
+            const double c = s_c[ee];
+            V rs, rd;
+            if (ch == 0) {
+              // integralKernelS (src/SU2/SU2FrgCore.cpp:199-227)
+              rs = V(-0.5) * As[0] * As[1] - V(0.5) * As[2] * As[3] + Ad[0] * As[1] + Ad[2] * As[3] + As[0] * Ad[1] +
+                   As[2] * Ad[3];
+              rd = V(0.1875) * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
+            } else if (ch == 2) {
+              // integralKernelU (:305-334)
+              rs = V(0.5) * As[0] * As[1] + V(0.5) * As[2] * As[3] + As[0] * Ad[1] + As[2] * Ad[3] + Ad[0] * As[1] +
+                   Ad[2] * As[3];
+              rd = V(0.1875) * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
+            } else {
+              // integralKernelT (:229-303) without the lattice bubble
+              const V *lc = s_loc + ee * 8;
+              const V Cbs = lc[0], Cbd = lc[1], ICas = lc[2], ICad = lc[3];
+              const V Cbs2 = lc[4], Cbd2 = lc[5], ICas2 = lc[6], ICad2 = lc[7];
+              rs = (V(0.25) * Cbs - Cbd) * As[0] + (V(0.25) * ICas - ICad) * As[1] + (V(0.25) * Cbs2 - Cbd2) * As[2] +
+                   (V(0.25) * ICas2 - ICad2) * As[3];
+              rd = -(Cbd + V(0.75) * Cbs) * Ad[0] - (ICad + V(0.75) * ICas) * Ad[1] -
+                   (Cbd2 + V(0.75) * Cbs2) * Ad[2] - (ICad2 + V(0.75) * ICas2) * Ad[3];
+              // lattice-bubble factors: P = c A2, Q = A3 per channel
+              s_P[(0 * nb + ee) * Lp + rid] = V(c) * As[2];
+              s_P[(1 * nb + ee) * Lp + rid] = V(c) * Ad[2];
+              s_Q[(0 * nb + ee) * Lp + rid] = As[3];
+              s_Q[(1 * nb + ee) * Lp + rid] = Ad[3];
+            }
+            if (MIXED) {
+              ksum_add(ks, float(c) * float(rs));
+              ksum_add(kd, float(c) * float(rd));
+            } else {
+              acc_s = fma(c, double(rs), acc_s);
+              acc_d = fma(c, double(rd), acc_d);
+            }
+          }
         }
       }
       __syncthreads();
       if (ch == 1) {
-        // (C) C[ch][a][b] += sum_e P[ch][e][a] Q[ch][e][b]
-        const int LL = L * L;
-        for (int j = tid; j < 2 * LL; j += kThreads) {
-          const int cch = j / LL;
-          const int r = j - cch * LL;
-          const int a = r / L, b = r - a * L;
-          const double *P = s_P + cch * nb * L + a;
-          const double *Q = s_Q + cch * nb * L + b;
-          double acc = s_C[j];
-          for (int e = 0; e < nvalid; ++e) acc = fma(P[e * L], Q[e * L], acc);
-          s_C[j] = acc;
+        // (C) C[ch][a][b] += sum_e P[ch][e][a] Q[ch][e][b], 2x2 register tiles
+        const int hp = Lp >> 1;
+        const int tiles = 2 * hp * hp;
+        for (int j = tid; j < tiles; j += kThreads) {
+          const int cch = j / (hp * hp);
+          const int r = j - cch * hp * hp;
+          const int a = 2 * (r / hp), b = 2 * (r - (r / hp) * hp);
+          const V *P = s_P + cch * nb * Lp + a;
+          const V *Q = s_Q + cch * nb * Lp + b;
+          double *C0 = s_C + cch * Lp * Lp + a * Lp + b;
+          double *C1 = s_C + cch * Lp * Lp + (a + 1) * Lp + b;
+          if (MIXED) {
+            KSum c00, c01, c10, c11;
+            __builtin_memcpy(&c00, C0, 8);
+            __builtin_memcpy(&c01, C0 + 1, 8);
+            __builtin_memcpy(&c10, C1, 8);
+            __builtin_memcpy(&c11, C1 + 1, 8);
+            for (int e = 0; e < nvalid; ++e) {
+              const V2 p = *reinterpret_cast<const V2 *>(P + e * Lp);
+              const V2 q = *reinterpret_cast<const V2 *>(Q + e * Lp);
+              ksum_add(c00, float(p.x) * float(q.x));
+              ksum_add(c01, float(p.x) * float(q.y));
+              ksum_add(c10, float(p.y) * float(q.x));
+              ksum_add(c11, float(p.y) * float(q.y));
+            }
+            __builtin_memcpy(C0, &c00, 8);
+            __builtin_memcpy(C0 + 1, &c01, 8);
+            __builtin_memcpy(C1, &c10, 8);
+            __builtin_memcpy(C1 + 1, &c11, 8);
+          } else {
+            double2 c0 = *reinterpret_cast<double2 *>(C0), c1 = *reinterpret_cast<double2 *>(C1);
+            for (int e = 0; e < nvalid; ++e) {
+              const V2 p = *reinterpret_cast<const V2 *>(P + e * Lp);
+              const V2 q = *reinterpret_cast<const V2 *>(Q + e * Lp);
+              c0.x = fma(double(p.x), double(q.x), c0.x);
+              c0.y = fma(double(p.x), double(q.y), c0.y);
+              c1.x = fma(double(p.y), double(q.x), c1.x);
+              c1.y = fma(double(p.y), double(q.y), c1.y);
+            }
+            *reinterpret_cast<double2 *>(C0) = c0;
+            *reinterpret_cast<double2 *>(C1) = c1;
+          }
         }
         __syncthreads();
       }
@@ -595,21 +790,33 @@ __global__ __launch_bounds__(kThreads) void k_dgamma(GammaArgs g) {
       const int wave = tid >> 6, lane = tid & 63;
       for (int r = wave; r < L; r += kThreads / 64) {
         double ps = 0.0, pd = 0.0;
-        const int b = g.ovl_off[r], e = g.ovl_off[r + 1];
+        const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
         for (int i = b + lane; i < e; i += 64) {
-          const int idx = g.ovl_rid1[i] * L + g.ovl_rid2[i];
-          ps += s_C[idx];
-          pd += s_C[L * L + idx];
+          const int idx = F.ovl_rid1[i] * Lp + F.ovl_rid2[i];
+          if (MIXED) {
+            KSum a0, a1;
+            __builtin_memcpy(&a0, s_C + idx, 8);
+            __builtin_memcpy(&a1, s_C + Lp * Lp + idx, 8);
+            ps += double(a0.s) + double(a0.c);
+            pd += double(a1.s) + double(a1.c);
+          } else {
+            ps += s_C[idx];
+            pd += s_C[Lp * Lp + idx];
+          }
         }
         ps = wave_sum(ps);
         pd = wave_sum(pd);
-        if (lane == 0) { s_rpa[2 * r] = g.two_s * ps; s_rpa[2 * r + 1] = g.eight_s * pd; }
+        if (lane == 0) { s_rpa[2 * r] = 2.0 * F.spin * ps; s_rpa[2 * r + 1] = 8.0 * F.spin * pd; }
       }
       __syncthreads();
     }
   }
 
   // reduce the slots; prefactor 1/(2 pi) (src/SU2/SU2FrgCore.cpp:426-430)
+  if (MIXED) {
+    acc_s = double(ks.s) + double(ks.c);
+    acc_d = double(kd.s) + double(kd.c);
+  }
   if (active) { s_red[(slot * L + rid) * 2] = acc_s; s_red[(slot * L + rid) * 2 + 1] = acc_d; }
   __syncthreads();
   if (tid < L) {
@@ -617,50 +824,69 @@ __global__ __launch_bounds__(kThreads) void k_dgamma(GammaArgs g) {
     for (int k = 0; k < slots; ++k) { ts += s_red[(k * L + tid) * 2]; td += s_red[(k * L + tid) * 2 + 1]; }
     ts *= (1.0 / kTwoPi);
     td *= (1.0 / kTwoPi);
-    g.dG[(size_t)item * L + tid] = make_double2(ts, td);
-    if (isnan(ts) || isnan(td)) atomicOr(&g.flags[0], 1);
+    F.dG[(size_t)item * L + tid] = make_double2(ts, td);
+    if (isnan(ts) || isnan(td)) atomicOr(&F.flags[0], 1);
   }
 }
 
-// in-place explicit Euler step (src/SU2/SU2FrgCore.cpp:111-137)
-__global__ void k_euler(double2 *__restrict__ G, const double2 *__restrict__ dG, size_t n, double *__restrict__ v2,
-                        const double *__restrict__ dv2, int nw, double step) {
+// in-place explicit Euler step (src/SU2/SU2FrgCore.cpp:111-137); a flow whose
+// derivative holds a NaN is left untouched, as the reference stops before
+// finalizeStep (src/SpinParser.cpp:151-159).  Rewrites the FP32 shadow.
+__global__ void k_euler(const FlowDev *__restrict__ fl) {
+  const FlowDev &F = fl[blockIdx.y];
+  if (!F.euler || F.flags[0] != 0) return;
+  const double step = F.step;
+  const size_t n = (size_t)F.nv;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  double2 *G = F.G;
+  const double2 *dG = F.dG;
+  float2 *G32 = F.G32;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double2 a = G[i];
     const double2 d = dG[i];
     a.x = fma(step, d.x, a.x);
     a.y = fma(step, d.y, a.y);
     G[i] = a;
+    if (G32) G32[i] = make_float2((float)a.x, (float)a.y);
   }
   if (blockIdx.x == 0)
-    for (int i = threadIdx.x; i < nw; i += blockDim.x) v2[i] = fma(step, dv2[i], v2[i]);
+    for (int i = threadIdx.x; i < F.nw; i += blockDim.x) F.v2[i] = fma(step, F.dv2[i], F.v2[i]);
+}
+
+// FP32 shadow of an uploaded state (mixed precision)
+__global__ void k_shadow(const FlowDev *__restrict__ fl) {
+  const FlowDev &F = fl[blockIdx.y];
+  if (!F.G32) return;
+  const size_t n = (size_t)F.nv;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    F.G32[i] = make_float2((float)F.G[i].x, (float)F.G[i].y);
 }
 
 // ---------------------------------------------------------------------------
 // static susceptibility (src/SU2/SU2MeasurementCorrelation.cpp:77-177), nu = 0
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_corr_partial(const double2 *__restrict__ G,
-                                                           const double *__restrict__ m, const double *__restrict__ v2,
-                                                           const int *__restrict__ site_rid,
-                                                           const int *__restrict__ inv_rid,
-                                                           const double *__restrict__ Cwp,
-                                                           const double *__restrict__ Cw, const int *__restrict__ Cn,
-                                                           int nw, int L, int slots, double spin, double *partial) {
+__global__ __launch_bounds__(kThreads) void k_corr_partial(const FlowDev *__restrict__ fl) {
   __shared__ double red[kThreads * 2];
+  const FlowDev &F = fl[blockIdx.y];
   const int o = blockIdx.x;
-  const int n = *Cn;
+  const int n = *F.Cn;
   if (o >= n) return;
+  const int nw = F.nw, L = F.L;
+  const double *m = F.mesh, *v2 = F.v2;
+  const double2 *G = F.G;
+  const double spin = F.spin;
+  const int slots = kThreads / L;
   const int tid = threadIdx.x;
   const int slot = tid / L, rid = tid - (tid / L) * L;
   const bool active = slot < slots;
   const double nu = 0.0;
-  const double w = Cwp[o];
+  const double w = F.Cwp[o];
   const double gw = (w + sigma_at(m, nw, v2, w)) * (w + nu + sigma_at(m, nw, v2, w + nu));
   double acc_s = 0.0, acc_d = 0.0;
   if (active) {
     for (int e = slot; e < n; e += slots) {
-      const double wp = Cwp[e];
+      const double wp = F.Cwp[e];
       // generateAccessBuffer(w + wp + nu, nu, w - wp): trilinear, 8 supports (:500-557)
       double a = w + wp + nu, b = nu, c = w - wp;
       bool exch = false;
@@ -675,7 +901,7 @@ __global__ __launch_bounds__(kThreads) void k_corr_partial(const double2 *__rest
       interp(m, nw, a, lS, uS, bS);
       interp(m, nw, b, lT, uT, bT);
       interp(m, nw, c, lU, uU, bU);
-      const int sr = exch ? inv_rid[rid] : site_rid[rid];
+      const int sr = exch ? F.inv_rid[rid] : F.site_rid[rid];
       const int S3[8] = {lS, uS, lS, uS, lS, uS, lS, uS};
       const int T3[8] = {lT, lT, uT, uT, lT, lT, uT, uT};
       const int U3[8] = {lU, lU, lU, lU, uU, uU, uU, uU};
@@ -700,7 +926,7 @@ __global__ __launch_bounds__(kThreads) void k_corr_partial(const double2 *__rest
       }
       const double norm = 1.0 / (gw * (wp + sigma_at(m, nw, v2, wp)) * (wp + nu + sigma_at(m, nw, v2, wp + nu)) *
                                  (4.0 * kPi * kPi));
-      const double cf = Cw[e] * norm;
+      const double cf = F.Cw[e] * norm;
       acc_s = fma(cf, rs, acc_s);
       acc_d = fma(cf, rd, acc_d);
     }
@@ -716,25 +942,30 @@ __global__ __launch_bounds__(kThreads) void k_corr_partial(const double2 *__rest
       ts += spin * term1 / kTwoPi;
       td += 2.0 * spin * term1 / kPi;
     }
-    partial[((size_t)o * L + tid) * 2] = Cw[o] * ts;
-    partial[((size_t)o * L + tid) * 2 + 1] = Cw[o] * td;
+    F.corr_partial[((size_t)o * L + tid) * 2] = F.Cw[o] * ts;
+    F.corr_partial[((size_t)o * L + tid) * 2 + 1] = F.Cw[o] * td;
   }
 }
 
-__global__ void k_corr_final(const double *__restrict__ partial, const int *__restrict__ Cn, int L,
-                             const int *__restrict__ corr_rid, int nout, double *out) {
+__global__ void k_corr_final(const FlowDev *__restrict__ fl) {
   __shared__ double susc[2 * 256];
-  const int n = *Cn;
+  const FlowDev &F = fl[blockIdx.y];
+  const int n = *F.Cn;
+  const int L = F.L;
   for (int r = threadIdx.x; r < L; r += blockDim.x) {
     double s = 0.0, d = 0.0;
-    for (int o = 0; o < n; ++o) { s += partial[((size_t)o * L + r) * 2]; d += partial[((size_t)o * L + r) * 2 + 1]; }
+    for (int o = 0; o < n; ++o) {
+      s += F.corr_partial[((size_t)o * L + r) * 2];
+      d += F.corr_partial[((size_t)o * L + r) * 2 + 1];
+    }
     susc[2 * r] = s;
     susc[2 * r + 1] = d;
   }
   __syncthreads();
+  const int nout = F.B * F.Rb;
   for (int k = threadIdx.x; k < nout; k += blockDim.x) {
-    out[k] = susc[2 * corr_rid[k]];
-    out[nout + k] = susc[2 * corr_rid[k] + 1];
+    F.corr_out[k] = susc[2 * F.corr_rid[k]];
+    F.corr_out[nout + k] = susc[2 * F.corr_rid[k] + 1];
   }
 }
 
@@ -746,29 +977,28 @@ __global__ void k_corr_final(const double *__restrict__ partial, const int *__re
 struct sp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  int nw = 0, L = 0, R = 0, B = 0, Rb = 0, O = 0, emax = 0;
-  long long nf = 0;
-  double spin = 0.5;
+  FlowDev fd{};             // host copy of the flow descriptor
+  FlowDev *d_fd = nullptr;  // device copy used by single-flow launches
   double cutoff = 0.0;
   bool have_flow = false;
-  int slots = 0, nb = 0;
   size_t lds = 0;
-  double *d_mesh = nullptr;
-  int *d_site = nullptr, *d_inv = nullptr, *d_ovl_off = nullptr, *d_ovl1 = nullptr, *d_ovl2 = nullptr;
-  int *d_rfwd = nullptr, *d_rinv = nullptr, *d_corr = nullptr;
-  double2 *d_G = nullptr, *d_dG = nullptr;
-  double *d_v2 = nullptr, *d_dv2 = nullptr;
-  double *d_Wwp = nullptr, *d_Wc = nullptr, *d_Cwp = nullptr, *d_Cw = nullptr;
-  int *d_Wn = nullptr, *d_Cn = nullptr;
-  double *d_partial = nullptr, *d_corr_out = nullptr;
-  int *d_flags = nullptr;
   std::vector<void *> allocs;
-  // profiling
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[4];
   std::vector<hipEvent_t> free_events;
   double prof_ms[4] = {0, 0, 0, 0};
   long prof_n[4] = {0, 0, 0, 0};
+  std::string err;
+};
+
+struct sp_group {
+  int capacity = 0;
+  hipStream_t stream = nullptr;
+  FlowDev *d_fd = nullptr;
+  double *d_out = nullptr;  // per flow: 2*B*Rb correlation values, then 2 flag slots
+  double *h_out = nullptr;  // pinned mirror of d_out
+  size_t out_cap = 0;       // doubles
+  std::vector<FlowDev> h_fd;
   std::string err;
 };
 
@@ -778,11 +1008,21 @@ int fail(sp_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
   return code;
 }
+int gfail(sp_group *g, int code, const std::string &msg) {
+  if (g) g->err = msg;
+  return code;
+}
 
-#define SP_HIP(call)                                                                          \
-  do {                                                                                        \
-    hipError_t e_ = (call);                                                                   \
+#define SP_HIP(call)                                                                                    \
+  do {                                                                                                  \
+    hipError_t e_ = (call);                                                                             \
     if (e_ != hipSuccess) return fail(ctx, SP_EHIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define SP_GHIP(call)                                                                                    \
+  do {                                                                                                   \
+    hipError_t e_ = (call);                                                                              \
+    if (e_ != hipSuccess) return gfail(grp, SP_EHIP, std::string(#call ": ") + hipGetErrorString(e_)); \
   } while (0)
 
 template <class T> int dalloc(sp_ctx *ctx, T **p, size_t count) {
@@ -811,15 +1051,15 @@ struct ProfScope {
   int k;
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(sp_ctx *c, int kernel) : ctx(c), k(kernel) {
-    if (ctx->prof) {
+    if (ctx && ctx->prof) {
       a = get_event(ctx);
       b = get_event(ctx);
-      if (a && b) hipEventRecord(a, ctx->stream);
+      if (a && b) (void)hipEventRecord(a, ctx->stream);
     }
   }
   ~ProfScope() {
-    if (ctx->prof && a && b) {
-      hipEventRecord(b, ctx->stream);
+    if (ctx && ctx->prof && a && b) {
+      (void)hipEventRecord(b, ctx->stream);
       ctx->pending[k].push_back({a, b});
     }
   }
@@ -848,27 +1088,102 @@ int check_problem(sp_ctx *ctx, const sp_su2_problem *p) {
       !in_range(p->ovl_rid2, O) || !in_range(p->range_fwd, p->R) || !in_range(p->range_inv, p->R) ||
       !in_range(p->corr_rid, (long)p->B * p->Rb))
     return fail(ctx, SP_EARG, "lattice table entry outside [0, L)");
-  if (p->precision != SP_PRECISION_FP64) return fail(ctx, SP_EARG, "unsupported precision mode");
+  if (p->precision != SP_PRECISION_FP64 && p->precision != SP_PRECISION_MIXED)
+    return fail(ctx, SP_EARG, "unsupported precision mode");
   return SP_OK;
 }
 
-void launch_shape(int nw, int L, int &slots, int &nb, size_t &lds) {
+void launch_shape(int nw, int L, int vs, int &slots, int &nb, size_t &lds) {
   slots = kThreads / L;
   int per = kMaxBatch / slots;
   if (per < 1) per = 1;
   nb = slots * per;
   if (nb > kMaxBatch) nb = kMaxBatch;
-  lds = (size_t)lds_layout(nw, L, nb, slots).total;
+  lds = (size_t)lds_layout(nw, L, nb, slots, vs).total;
 }
 
-int weights(sp_ctx *ctx, int corr_only) {
-  ProfScope ps(ctx, 2);
-  const int threads = 64;
-  const int blocks = (ctx->nw + 1 + threads - 1) / threads;
-  hipLaunchKernelGGL(k_weights, dim3(blocks), dim3(threads), 0, ctx->stream, ctx->d_mesh, ctx->nw, ctx->d_v2,
-                     ctx->d_dv2, ctx->cutoff, ctx->emax, ctx->d_Wwp, ctx->d_Wc, ctx->d_Wn, ctx->d_Cwp, ctx->d_Cw,
-                     ctx->d_Cn, corr_only, ctx->d_flags);
-  SP_HIP(hipGetLastError());
+size_t lds_of(const FlowDev &f) {
+  return (size_t)lds_layout(f.nw, f.L, f.nb, f.slots, f.precision == SP_PRECISION_MIXED ? 4 : 8).total;
+}
+
+// Launch geometry of a set of flows held in a host array (h) whose device copy is d.
+struct Geometry {
+  int maxnw = 0, maxemax = 0;
+  long long maxnv = 0;
+};
+
+Geometry geometry(const FlowDev *h, int n) {
+  Geometry g;
+  for (int i = 0; i < n; ++i) {
+    g.maxnw = std::max(g.maxnw, h[i].nw);
+    g.maxemax = std::max(g.maxemax, h[i].emax);
+    g.maxnv = std::max(g.maxnv, h[i].nv);
+  }
+  return g;
+}
+
+// dSigma, weight tables and dGamma for n flows; h[i].item_base must be set
+// relative to the first flow of each precision class, FP64 flows first.
+hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, int n, sp_ctx *prof) {
+  const Geometry geo = geometry(h, n);
+  {
+    ProfScope ps(prof, 1);
+    hipLaunchKernelGGL(k_dsigma, dim3(geo.maxnw, n), dim3(64), 0, st, d);
+  }
+  {
+    ProfScope ps(prof, 2);
+    hipLaunchKernelGGL(k_weights, dim3((geo.maxnw + 64) / 64, n), dim3(64), 0, st, d, 0);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  ProfScope ps(prof, 0);
+  int i0 = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool mixed = pass == 1;
+    int i1 = i0;
+    long long total = 0;
+    size_t lds = 0;
+    while (i1 < n && (h[i1].precision == SP_PRECISION_MIXED) == mixed) {
+      total += h[i1].nf;
+      lds = std::max(lds, lds_of(h[i1]));
+      ++i1;
+    }
+    if (i1 > i0) {
+      const long long per_xcd = 0;  // unused by the chunked mapping
+      const long long rounds = (total + 8 * kChunk - 1) / (8 * kChunk);
+      const unsigned grid = (unsigned)(rounds * 8 * kChunk);
+      if (mixed)
+        hipLaunchKernelGGL(k_dgamma<true>, dim3(grid), dim3(kThreads), lds, st, d + i0, i1 - i0, total, per_xcd);
+      else
+        hipLaunchKernelGGL(k_dgamma<false>, dim3(grid), dim3(kThreads), lds, st, d + i0, i1 - i0, total, per_xcd);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    i0 = i1;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_measure(hipStream_t st, const FlowDev *d, const FlowDev *h, int n) {
+  const Geometry geo = geometry(h, n);
+  hipLaunchKernelGGL(k_weights, dim3((geo.maxnw + 64) / 64, n), dim3(64), 0, st, d, 1);
+  hipLaunchKernelGGL(k_corr_partial, dim3(geo.maxemax, n), dim3(kThreads), 0, st, d);
+  hipLaunchKernelGGL(k_corr_final, dim3(1, n), dim3(256), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_euler(hipStream_t st, const FlowDev *d, const FlowDev *h, int n, sp_ctx *prof) {
+  const Geometry geo = geometry(h, n);
+  long long blocks = (geo.maxnv + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  ProfScope ps(prof, 3);
+  hipLaunchKernelGGL(k_euler, dim3((unsigned)blocks, n), dim3(256), 0, st, d);
+  return hipGetLastError();
+}
+
+int upload_fd(sp_ctx *ctx) {
+  SP_HIP(hipMemcpyAsync(ctx->d_fd, &ctx->fd, sizeof(FlowDev), hipMemcpyHostToDevice, ctx->stream));
   return SP_OK;
 }
 
@@ -880,11 +1195,13 @@ int sp_ctx_footprint(const sp_su2_problem *p, long long *bytes) {
   if (!p || !bytes || p->nw < 2 || p->L < 1) return SP_EARG;
   const long long nf = (long long)p->nw * p->nw * (p->nw + 1) / 2;
   const long long emax = 2LL * p->nw + 8;
-  long long b = 2 * nf * p->L * 16;            // Gamma, dGamma
-  b += 2LL * p->nw * 8 + p->nw * 8;            // Sigma, dSigma, mesh
+  long long b = 2 * nf * p->L * 16;  // Gamma, dGamma
+  if (p->precision == SP_PRECISION_MIXED) b += nf * p->L * 8;  // FP32 shadow
+  b += 3LL * p->nw * 8;                                          // Sigma, dSigma, mesh
   b += 2 * (long long)p->nw * emax * 8 + 2 * emax * 8 + (p->nw + 1) * 4;  // weight tables
-  b += emax * p->L * 16 + 2LL * p->B * p->Rb * 8;  // correlation scratch / output
+  b += emax * p->L * 16 + 2LL * p->B * p->Rb * 8 + 16;                   // correlation scratch / output, flags
   b += 4LL * (3 * p->L + 1 + 2 * (p->ovl_off ? p->ovl_off[p->L] : 0) + 2 * p->R + p->B * p->Rb);
+  b += (long long)sizeof(FlowDev);
   *bytes = b;
   return SP_OK;
 }
@@ -894,69 +1211,93 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   *out = nullptr;
   sp_ctx *ctx = new (std::nothrow) sp_ctx();
   if (!ctx) return SP_ENOMEM;
-  int rc = check_problem(ctx, p);
-  if (rc != SP_OK) { *out = ctx; return rc; }
   *out = ctx;
+  int rc = check_problem(ctx, p);
+  if (rc != SP_OK) return rc;
   SP_HIP(hipGetDevice(&ctx->device));
   SP_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-  ctx->nw = p->nw;
-  ctx->L = p->L;
-  ctx->R = p->R;
-  ctx->B = p->B;
-  ctx->Rb = p->Rb;
-  ctx->O = p->ovl_off[p->L];
-  ctx->nf = (long long)p->nw * p->nw * (p->nw + 1) / 2;
-  ctx->emax = 2 * p->nw + 8;
-  ctx->spin = p->spin;
-  launch_shape(ctx->nw, ctx->L, ctx->slots, ctx->nb, ctx->lds);
-  if (ctx->lds > 64 * 1024) return fail(ctx, SP_EARG, "problem too large for the LDS plan of the dGamma kernel");
-  if (ctx->nf * ctx->L >= (1LL << 31) / 2) return fail(ctx, SP_EARG, "vertex too large for 32-bit row indices");
-
-  const size_t nv = (size_t)ctx->nf * ctx->L;
-  const int emax = ctx->emax;
-  if ((rc = dalloc(ctx, &ctx->d_mesh, ctx->nw)) || (rc = dalloc(ctx, &ctx->d_site, ctx->L)) ||
-      (rc = dalloc(ctx, &ctx->d_inv, ctx->L)) || (rc = dalloc(ctx, &ctx->d_ovl_off, ctx->L + 1)) ||
-      (rc = dalloc(ctx, &ctx->d_ovl1, ctx->O)) || (rc = dalloc(ctx, &ctx->d_ovl2, ctx->O)) ||
-      (rc = dalloc(ctx, &ctx->d_rfwd, ctx->R)) || (rc = dalloc(ctx, &ctx->d_rinv, ctx->R)) ||
-      (rc = dalloc(ctx, &ctx->d_corr, (size_t)ctx->B * ctx->Rb)) || (rc = dalloc(ctx, &ctx->d_G, nv)) ||
-      (rc = dalloc(ctx, &ctx->d_dG, nv)) || (rc = dalloc(ctx, &ctx->d_v2, ctx->nw)) ||
-      (rc = dalloc(ctx, &ctx->d_dv2, ctx->nw)) || (rc = dalloc(ctx, &ctx->d_Wwp, (size_t)ctx->nw * emax)) ||
-      (rc = dalloc(ctx, &ctx->d_Wc, (size_t)ctx->nw * emax)) || (rc = dalloc(ctx, &ctx->d_Wn, ctx->nw)) ||
-      (rc = dalloc(ctx, &ctx->d_Cwp, emax)) || (rc = dalloc(ctx, &ctx->d_Cw, emax)) ||
-      (rc = dalloc(ctx, &ctx->d_Cn, 1)) || (rc = dalloc(ctx, &ctx->d_partial, (size_t)emax * ctx->L * 2)) ||
-      (rc = dalloc(ctx, &ctx->d_corr_out, (size_t)2 * ctx->B * ctx->Rb)) || (rc = dalloc(ctx, &ctx->d_flags, 4)))
+  FlowDev &f = ctx->fd;
+  f.nw = p->nw;
+  f.L = p->L;
+  f.R = p->R;
+  f.B = p->B;
+  f.Rb = p->Rb;
+  f.nf = (long long)p->nw * p->nw * (p->nw + 1) / 2;
+  f.nv = f.nf * p->L;
+  f.emax = 2 * p->nw + 8;
+  f.spin = p->spin;
+  f.precision = p->precision;
+  f.item_base = 0;
+  launch_shape(f.nw, f.L, f.precision == SP_PRECISION_MIXED ? 4 : 8, f.slots, f.nb, ctx->lds);
+  int max_lds = 0;
+  SP_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  if (ctx->lds > (size_t)max_lds) return fail(ctx, SP_EARG, "problem too large for the LDS plan of the dGamma kernel");
+  if (f.nv * 16 >= (1LL << 31)) return fail(ctx, SP_EARG, "vertex too large for 32-bit byte offsets");
+  f.g_bytes = (unsigned)(f.nv * 16);
+  f.g32_bytes = (unsigned)(f.nv * 8);
+  const int O = p->ovl_off[p->L];
+  const size_t nv = (size_t)f.nv;
+  const int emax = f.emax;
+  double *mesh = nullptr;
+  int *site = nullptr, *inv = nullptr, *ooff = nullptr, *o1 = nullptr, *o2 = nullptr, *rf = nullptr, *ri = nullptr,
+      *corr = nullptr;
+  if ((rc = dalloc(ctx, &mesh, f.nw)) || (rc = dalloc(ctx, &site, f.L)) || (rc = dalloc(ctx, &inv, f.L)) ||
+      (rc = dalloc(ctx, &ooff, f.L + 1)) || (rc = dalloc(ctx, &o1, O)) || (rc = dalloc(ctx, &o2, O)) ||
+      (rc = dalloc(ctx, &rf, f.R)) || (rc = dalloc(ctx, &ri, f.R)) || (rc = dalloc(ctx, &corr, (size_t)f.B * f.Rb)) ||
+      (rc = dalloc(ctx, &f.G, nv)) || (rc = dalloc(ctx, &f.dG, nv)) || (rc = dalloc(ctx, &f.v2, f.nw)) ||
+      (rc = dalloc(ctx, &f.dv2, f.nw)) || (rc = dalloc(ctx, &f.Wwp, (size_t)f.nw * emax)) ||
+      (rc = dalloc(ctx, &f.Wc, (size_t)f.nw * emax)) || (rc = dalloc(ctx, &f.Wn, f.nw)) ||
+      (rc = dalloc(ctx, &f.Cwp, emax)) || (rc = dalloc(ctx, &f.Cw, emax)) || (rc = dalloc(ctx, &f.Cn, 1)) ||
+      (rc = dalloc(ctx, &f.corr_partial, (size_t)emax * f.L * 2)) ||
+      (rc = dalloc(ctx, &f.corr_out, (size_t)2 * f.B * f.Rb)) || (rc = dalloc(ctx, &f.flags, 4)) ||
+      (rc = dalloc(ctx, &ctx->d_fd, 1)))
     return rc;
+  if (f.precision == SP_PRECISION_MIXED && (rc = dalloc(ctx, &f.G32, nv))) return rc;
+  f.mesh = mesh;
+  f.site_rid = site;
+  f.inv_rid = inv;
+  f.ovl_off = ooff;
+  f.ovl_rid1 = o1;
+  f.ovl_rid2 = o2;
+  f.range_fwd = rf;
+  f.range_inv = ri;
+  f.corr_rid = corr;
 
-  std::vector<double> mesh(ctx->nw);
-  for (int i = 0; i < ctx->nw; ++i) mesh[i] = (double)p->mesh[i];
-  SP_HIP(hipMemcpy(ctx->d_mesh, mesh.data(), ctx->nw * 8, hipMemcpyHostToDevice));
-  SP_HIP(hipMemcpy(ctx->d_site, p->site_rid, ctx->L * 4, hipMemcpyHostToDevice));
-  SP_HIP(hipMemcpy(ctx->d_inv, p->inv_rid, ctx->L * 4, hipMemcpyHostToDevice));
-  SP_HIP(hipMemcpy(ctx->d_ovl_off, p->ovl_off, (ctx->L + 1) * 4, hipMemcpyHostToDevice));
-  if (ctx->O > 0) {
-    SP_HIP(hipMemcpy(ctx->d_ovl1, p->ovl_rid1, (size_t)ctx->O * 4, hipMemcpyHostToDevice));
-    SP_HIP(hipMemcpy(ctx->d_ovl2, p->ovl_rid2, (size_t)ctx->O * 4, hipMemcpyHostToDevice));
+  std::vector<double> m(f.nw);
+  for (int i = 0; i < f.nw; ++i) m[i] = (double)p->mesh[i];
+  SP_HIP(hipMemcpy(mesh, m.data(), f.nw * 8, hipMemcpyHostToDevice));
+  SP_HIP(hipMemcpy(site, p->site_rid, f.L * 4, hipMemcpyHostToDevice));
+  SP_HIP(hipMemcpy(inv, p->inv_rid, f.L * 4, hipMemcpyHostToDevice));
+  SP_HIP(hipMemcpy(ooff, p->ovl_off, (f.L + 1) * 4, hipMemcpyHostToDevice));
+  if (O > 0) {
+    SP_HIP(hipMemcpy(o1, p->ovl_rid1, (size_t)O * 4, hipMemcpyHostToDevice));
+    SP_HIP(hipMemcpy(o2, p->ovl_rid2, (size_t)O * 4, hipMemcpyHostToDevice));
   }
-  SP_HIP(hipMemcpy(ctx->d_rfwd, p->range_fwd, (size_t)ctx->R * 4, hipMemcpyHostToDevice));
-  SP_HIP(hipMemcpy(ctx->d_rinv, p->range_inv, (size_t)ctx->R * 4, hipMemcpyHostToDevice));
-  SP_HIP(hipMemcpy(ctx->d_corr, p->corr_rid, (size_t)ctx->B * ctx->Rb * 4, hipMemcpyHostToDevice));
-  SP_HIP(hipMemset(ctx->d_G, 0, nv * sizeof(double2)));
-  SP_HIP(hipMemset(ctx->d_dG, 0, nv * sizeof(double2)));
-  SP_HIP(hipMemset(ctx->d_v2, 0, ctx->nw * 8));
-  SP_HIP(hipMemset(ctx->d_dv2, 0, ctx->nw * 8));
-  SP_HIP(hipMemset(ctx->d_flags, 0, 16));
-  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+  SP_HIP(hipMemcpy(rf, p->range_fwd, (size_t)f.R * 4, hipMemcpyHostToDevice));
+  SP_HIP(hipMemcpy(ri, p->range_inv, (size_t)f.R * 4, hipMemcpyHostToDevice));
+  SP_HIP(hipMemcpy(corr, p->corr_rid, (size_t)f.B * f.Rb * 4, hipMemcpyHostToDevice));
+  SP_HIP(hipMemset(f.G, 0, nv * sizeof(double2)));
+  SP_HIP(hipMemset(f.dG, 0, nv * sizeof(double2)));
+  if (f.G32) SP_HIP(hipMemset(f.G32, 0, nv * sizeof(float2)));
+  SP_HIP(hipMemset(f.v2, 0, f.nw * 8));
+  SP_HIP(hipMemset(f.dv2, 0, f.nw * 8));
+  SP_HIP(hipMemset(f.flags, 0, 16));
+  SP_HIP(hipMemset(f.Wn, 0, f.nw * 4));
+  SP_HIP(hipMemset(f.Cn, 0, 4));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_HIP(hipMemcpy(ctx->d_fd, &f, sizeof(FlowDev), hipMemcpyHostToDevice));
   return SP_OK;
 }
 
 int sp_ctx_destroy(sp_ctx *ctx) {
   if (!ctx) return SP_OK;
-  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (int k = 0; k < 4; ++k)
-    for (auto &pr : ctx->pending[k]) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
-  for (auto e : ctx->free_events) hipEventDestroy(e);
-  for (void *q : ctx->allocs) hipFree(q);
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+    for (auto &pr : ctx->pending[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+  for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+  for (void *q : ctx->allocs) (void)hipFree(q);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SP_OK;
 }
@@ -964,20 +1305,28 @@ int sp_ctx_destroy(sp_ctx *ctx) {
 int sp_set_state(sp_ctx *ctx, double cutoff, const double *v2, const double *ss, const double *dd) {
   if (!ctx || !v2 || !ss || !dd) return fail(ctx, SP_EARG, "null argument");
   if (!std::isfinite(cutoff) || cutoff <= 0.0) return fail(ctx, SP_EARG, "cutoff must be finite and positive");
-  const size_t nv = (size_t)ctx->nf * ctx->L;
+  FlowDev &f = ctx->fd;
+  const size_t nv = (size_t)f.nv;
   std::vector<double2> buf;
   try { buf.resize(nv); } catch (...) { return fail(ctx, SP_ENOMEM, "host buffer"); }
   for (size_t i = 0; i < nv; ++i) buf[i] = make_double2(ss[i], dd[i]);
-  SP_HIP(hipMemcpyAsync(ctx->d_G, buf.data(), nv * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
-  SP_HIP(hipMemcpyAsync(ctx->d_v2, v2, ctx->nw * 8, hipMemcpyHostToDevice, ctx->stream));
-  SP_HIP(hipStreamSynchronize(ctx->stream));
+  SP_HIP(hipMemcpyAsync(f.G, buf.data(), nv * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+  SP_HIP(hipMemcpyAsync(f.v2, v2, f.nw * 8, hipMemcpyHostToDevice, ctx->stream));
   ctx->cutoff = cutoff;
+  f.cut = cutoff;
+  int rc = upload_fd(ctx);
+  if (rc) return rc;
+  if (f.G32) {
+    hipLaunchKernelGGL(k_shadow, dim3(2048, 1), dim3(256), 0, ctx->stream, ctx->d_fd);
+    SP_HIP(hipGetLastError());
+  }
+  SP_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_flow = false;
   return SP_OK;
 }
 
 static int download_pair(sp_ctx *ctx, const double2 *src, double *a, double *b) {
-  const size_t nv = (size_t)ctx->nf * ctx->L;
+  const size_t nv = (size_t)ctx->fd.nv;
   std::vector<double2> buf;
   try { buf.resize(nv); } catch (...) { return fail(ctx, SP_ENOMEM, "host buffer"); }
   SP_HIP(hipMemcpyAsync(buf.data(), src, nv * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
@@ -993,10 +1342,10 @@ int sp_get_state(sp_ctx *ctx, double *cutoff, double *v2, double *ss, double *dd
   if (!ctx) return SP_EARG;
   if (cutoff) *cutoff = ctx->cutoff;
   if (v2) {
-    SP_HIP(hipMemcpyAsync(v2, ctx->d_v2, ctx->nw * 8, hipMemcpyDeviceToHost, ctx->stream));
+    SP_HIP(hipMemcpyAsync(v2, ctx->fd.v2, ctx->fd.nw * 8, hipMemcpyDeviceToHost, ctx->stream));
     SP_HIP(hipStreamSynchronize(ctx->stream));
   }
-  if (ss || dd) return download_pair(ctx, ctx->d_G, ss, dd);
+  if (ss || dd) return download_pair(ctx, ctx->fd.G, ss, dd);
   return SP_OK;
 }
 
@@ -1004,55 +1353,21 @@ int sp_get_flow(sp_ctx *ctx, double *dv2, double *dss, double *ddd) {
   if (!ctx) return SP_EARG;
   if (!ctx->have_flow) return fail(ctx, SP_ESTATE, "no flow computed since the last state change");
   if (dv2) {
-    SP_HIP(hipMemcpyAsync(dv2, ctx->d_dv2, ctx->nw * 8, hipMemcpyDeviceToHost, ctx->stream));
+    SP_HIP(hipMemcpyAsync(dv2, ctx->fd.dv2, ctx->fd.nw * 8, hipMemcpyDeviceToHost, ctx->stream));
     SP_HIP(hipStreamSynchronize(ctx->stream));
   }
-  if (dss || ddd) return download_pair(ctx, ctx->d_dG, dss, ddd);
+  if (dss || ddd) return download_pair(ctx, ctx->fd.dG, dss, ddd);
   return SP_OK;
 }
 
 int sp_compute_step(sp_ctx *ctx) {
   if (!ctx) return SP_EARG;
-  int rc;
-  SP_HIP(hipMemsetAsync(ctx->d_flags, 0, 16, ctx->stream));
-  {
-    ProfScope ps(ctx, 1);
-    hipLaunchKernelGGL(k_dsigma, dim3(ctx->nw), dim3(64), 0, ctx->stream, ctx->d_G, ctx->d_mesh, ctx->d_v2,
-                       ctx->d_rfwd, ctx->d_rinv, ctx->R, ctx->nw, ctx->L, ctx->cutoff, ctx->spin, ctx->d_dv2,
-                       ctx->d_flags);
-    SP_HIP(hipGetLastError());
-  }
-  if ((rc = weights(ctx, 0)) != SP_OK) return rc;
-  GammaArgs g;
-  g.G = ctx->d_G;
-  g.dG = ctx->d_dG;
-  g.mesh = ctx->d_mesh;
-  g.site_rid = ctx->d_site;
-  g.inv_rid = ctx->d_inv;
-  g.ovl_off = ctx->d_ovl_off;
-  g.ovl_rid1 = ctx->d_ovl1;
-  g.ovl_rid2 = ctx->d_ovl2;
-  g.Wwp = ctx->d_Wwp;
-  g.Wc = ctx->d_Wc;
-  g.Wn = ctx->d_Wn;
-  g.flags = ctx->d_flags;
-  g.nw = ctx->nw;
-  g.L = ctx->L;
-  g.emax = ctx->emax;
-  g.nb = ctx->nb;
-  g.slots = ctx->slots;
-  g.two_s = 2.0 * ctx->spin;
-  g.eight_s = 8.0 * ctx->spin;
-  {
-    ProfScope ps(ctx, 0);
-    const long long chunk = 1LL << 30;
-    for (long long i0 = 0; i0 < ctx->nf; i0 += chunk) {
-      const long long n = ctx->nf - i0 < chunk ? ctx->nf - i0 : chunk;
-      g.item0 = (int)i0;
-      hipLaunchKernelGGL(k_dgamma, dim3((unsigned)n), dim3(kThreads), ctx->lds, ctx->stream, g);
-      SP_HIP(hipGetLastError());
-    }
-  }
+  FlowDev &f = ctx->fd;
+  f.cut = ctx->cutoff;
+  int rc = upload_fd(ctx);
+  if (rc) return rc;
+  SP_HIP(hipMemsetAsync(f.flags, 0, 16, ctx->stream));
+  SP_HIP(launch_compute(ctx->stream, ctx->d_fd, &f, 1, ctx));
   ctx->have_flow = true;
   return SP_OK;
 }
@@ -1061,16 +1376,17 @@ int sp_finalize_step(sp_ctx *ctx, double new_cutoff) {
   if (!ctx) return SP_EARG;
   if (!ctx->have_flow) return fail(ctx, SP_ESTATE, "finalize_step without a computed flow");
   if (!std::isfinite(new_cutoff) || new_cutoff <= 0.0) return fail(ctx, SP_EARG, "cutoff must be finite and positive");
-  const double step = new_cutoff - ctx->cutoff;
-  const size_t nv = (size_t)ctx->nf * ctx->L;
-  {
-    ProfScope ps(ctx, 3);
-    size_t blocks = (nv + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_euler, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, ctx->d_G, ctx->d_dG, nv,
-                       ctx->d_v2, ctx->d_dv2, ctx->nw, step);
-    SP_HIP(hipGetLastError());
-  }
+  FlowDev &f = ctx->fd;
+  f.step = new_cutoff - ctx->cutoff;
+  f.euler = 1;
+  f.cut = new_cutoff;
+  int rc = upload_fd(ctx);
+  if (rc) return rc;
+  // the reference updates even a flow the caller did not check for NaN; the
+  // kernel only skips on the flag, so clear it for this explicit call
+  SP_HIP(hipMemsetAsync(f.flags, 0, 4, ctx->stream));
+  SP_HIP(launch_euler(ctx->stream, ctx->d_fd, &f, 1, ctx));
+  f.euler = 0;
   ctx->cutoff = new_cutoff;
   ctx->have_flow = false;
   return SP_OK;
@@ -1080,7 +1396,7 @@ int sp_is_diverged(sp_ctx *ctx, int *diverged) {
   if (!ctx || !diverged) return SP_EARG;
   if (!ctx->have_flow) return fail(ctx, SP_ESTATE, "no flow computed");
   int flags[4];
-  SP_HIP(hipMemcpyAsync(flags, ctx->d_flags, 16, hipMemcpyDeviceToHost, ctx->stream));
+  SP_HIP(hipMemcpyAsync(flags, ctx->fd.flags, 16, hipMemcpyDeviceToHost, ctx->stream));
   SP_HIP(hipStreamSynchronize(ctx->stream));
   if (flags[1]) return fail(ctx, SP_EARG, "evaluation list overflow");
   *diverged = flags[0] ? 1 : 0;
@@ -1089,18 +1405,14 @@ int sp_is_diverged(sp_ctx *ctx, int *diverged) {
 
 int sp_measure_correlation(sp_ctx *ctx, double *zz, double *dd) {
   if (!ctx || !zz || !dd) return SP_EARG;
-  int rc;
-  if ((rc = weights(ctx, 1)) != SP_OK) return rc;
-  hipLaunchKernelGGL(k_corr_partial, dim3(ctx->emax), dim3(kThreads), 0, ctx->stream, ctx->d_G, ctx->d_mesh,
-                     ctx->d_v2, ctx->d_site, ctx->d_inv, ctx->d_Cwp, ctx->d_Cw, ctx->d_Cn, ctx->nw, ctx->L,
-                     kThreads / ctx->L, ctx->spin, ctx->d_partial);
-  SP_HIP(hipGetLastError());
-  const int nout = ctx->B * ctx->Rb;
-  hipLaunchKernelGGL(k_corr_final, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, ctx->d_Cn, ctx->L,
-                     ctx->d_corr, nout, ctx->d_corr_out);
-  SP_HIP(hipGetLastError());
+  FlowDev &f = ctx->fd;
+  f.cut = ctx->cutoff;
+  int rc = upload_fd(ctx);
+  if (rc) return rc;
+  SP_HIP(launch_measure(ctx->stream, ctx->d_fd, &f, 1));
+  const int nout = f.B * f.Rb;
   std::vector<double> buf(2 * (size_t)nout);
-  SP_HIP(hipMemcpyAsync(buf.data(), ctx->d_corr_out, 2 * (size_t)nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+  SP_HIP(hipMemcpyAsync(buf.data(), f.corr_out, 2 * (size_t)nout * 8, hipMemcpyDeviceToHost, ctx->stream));
   SP_HIP(hipStreamSynchronize(ctx->stream));
   std::memcpy(zz, buf.data(), (size_t)nout * 8);
   std::memcpy(dd, buf.data() + nout, (size_t)nout * 8);
@@ -1135,6 +1447,157 @@ int sp_profile_get(sp_ctx *ctx, int kernel, double *total_ms, long *launches) {
   ctx->pending[kernel].clear();
   if (total_ms) *total_ms = ctx->prof_ms[kernel];
   if (launches) *launches = ctx->prof_n[kernel];
+  return SP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// flow groups: one launch per kernel for many resident flows
+// ---------------------------------------------------------------------------
+int sp_group_create(int capacity, sp_group **out) {
+  if (!out || capacity < 1 || capacity > 65535) return SP_EARG;
+  *out = nullptr;
+  sp_group *grp = new (std::nothrow) sp_group();
+  if (!grp) return SP_ENOMEM;
+  *out = grp;
+  grp->capacity = capacity;
+  grp->h_fd.resize(capacity);
+  SP_GHIP(hipStreamCreateWithFlags(&grp->stream, hipStreamNonBlocking));
+  void *q = nullptr;
+  if (hipMalloc(&q, sizeof(FlowDev) * capacity) != hipSuccess) return gfail(grp, SP_ENOMEM, "hipMalloc");
+  grp->d_fd = static_cast<FlowDev *>(q);
+  return SP_OK;
+}
+
+int sp_group_destroy(sp_group *grp) {
+  if (!grp) return SP_OK;
+  if (grp->stream) (void)hipStreamSynchronize(grp->stream);
+  if (grp->d_fd) (void)hipFree(grp->d_fd);
+  if (grp->d_out) (void)hipFree(grp->d_out);
+  if (grp->h_out) (void)hipHostFree(grp->h_out);
+  if (grp->stream) (void)hipStreamDestroy(grp->stream);
+  delete grp;
+  return SP_OK;
+}
+
+const char *sp_group_last_error(const sp_group *grp) { return grp ? grp->err.c_str() : "null group"; }
+
+static int group_prepare(sp_group *grp, sp_ctx *const *ctxs, int n, size_t *out_off) {
+  if (n < 1 || n > grp->capacity || !ctxs) return gfail(grp, SP_EARG, "invalid flow count");
+  // FP64 flows first, then mixed; item bases restart per precision class
+  std::vector<int> order;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < n; ++i)
+      if (ctxs[i] && (ctxs[i]->fd.precision == SP_PRECISION_MIXED) == (pass == 1)) order.push_back(i);
+  if ((int)order.size() != n) return gfail(grp, SP_EARG, "null context in group");
+  size_t need = 0;
+  for (int i = 0; i < n; ++i) {
+    out_off[i] = need;
+    need += 2 * (size_t)ctxs[i]->fd.B * ctxs[i]->fd.Rb + 2;
+  }
+  if (need > grp->out_cap) {
+    if (grp->d_out) (void)hipFree(grp->d_out);
+    if (grp->h_out) (void)hipHostFree(grp->h_out);
+    grp->d_out = nullptr;
+    grp->h_out = nullptr;
+    void *q = nullptr;
+    if (hipMalloc(&q, need * 8) != hipSuccess) return gfail(grp, SP_ENOMEM, "hipMalloc");
+    grp->d_out = static_cast<double *>(q);
+    if (hipHostMalloc(&q, need * 8, 0) != hipSuccess) return gfail(grp, SP_ENOMEM, "hipHostMalloc");
+    grp->h_out = static_cast<double *>(q);
+    grp->out_cap = need;
+  }
+  long long base = 0;
+  int prev = -1;
+  for (int j = 0; j < n; ++j) {
+    sp_ctx *c = ctxs[order[j]];
+    SP_GHIP(hipStreamSynchronize(c->stream));
+    FlowDev f = c->fd;
+    if (f.precision != prev) { base = 0; prev = f.precision; }
+    f.item_base = base;
+    base += f.nf;
+    f.cut = c->cutoff;
+    f.euler = 0;
+    f.corr_out = grp->d_out + out_off[order[j]];
+    grp->h_fd[j] = f;
+  }
+  // remember the order for the caller
+  for (int j = 0; j < n; ++j) grp->h_fd[j].pad_index = order[j];
+  return SP_OK;
+}
+
+int sp_group_step(sp_group *grp, sp_ctx *const *ctxs, int n, const double *new_cutoffs, double *const *zz,
+                  double *const *dd, int *diverged) {
+  if (!grp) return SP_EARG;
+  std::vector<size_t> off(n > 0 ? n : 1);
+  int rc = group_prepare(grp, ctxs, n, off.data());
+  if (rc) return rc;
+  for (int j = 0; j < n; ++j) {
+    sp_ctx *c = ctxs[grp->h_fd[j].pad_index];
+    SP_GHIP(hipMemsetAsync(c->fd.flags, 0, 16, grp->stream));
+  }
+  SP_GHIP(hipMemcpyAsync(grp->d_fd, grp->h_fd.data(), sizeof(FlowDev) * n, hipMemcpyHostToDevice, grp->stream));
+  SP_GHIP(launch_compute(grp->stream, grp->d_fd, grp->h_fd.data(), n, nullptr));
+  SP_GHIP(launch_measure(grp->stream, grp->d_fd, grp->h_fd.data(), n));
+  // divergence flags next to each flow's correlation record
+  for (int j = 0; j < n; ++j) {
+    const int i = grp->h_fd[j].pad_index;
+    const FlowDev &f = grp->h_fd[j];
+    SP_GHIP(hipMemcpyAsync(grp->d_out + off[i] + 2 * (size_t)f.B * f.Rb, f.flags, 8, hipMemcpyDeviceToDevice,
+                           grp->stream));
+  }
+  if (new_cutoffs) {
+    for (int j = 0; j < n; ++j) {
+      const int i = grp->h_fd[j].pad_index;
+      const double nc = new_cutoffs[i];
+      FlowDev &f = grp->h_fd[j];
+      if (std::isfinite(nc) && nc > 0.0) {
+        f.euler = 1;
+        f.step = nc - ctxs[i]->cutoff;
+      }
+    }
+    SP_GHIP(hipMemcpyAsync(grp->d_fd, grp->h_fd.data(), sizeof(FlowDev) * n, hipMemcpyHostToDevice, grp->stream));
+    SP_GHIP(launch_euler(grp->stream, grp->d_fd, grp->h_fd.data(), n, nullptr));
+  }
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total = std::max(total, off[i] + 2 * (size_t)ctxs[i]->fd.B * ctxs[i]->fd.Rb + 2);
+  SP_GHIP(hipMemcpyAsync(grp->h_out, grp->d_out, total * 8, hipMemcpyDeviceToHost, grp->stream));
+  SP_GHIP(hipStreamSynchronize(grp->stream));
+  for (int j = 0; j < n; ++j) {
+    const int i = grp->h_fd[j].pad_index;
+    sp_ctx *c = ctxs[i];
+    const size_t nout = (size_t)c->fd.B * c->fd.Rb;
+    const double *rec = grp->h_out + off[i];
+    if (zz && zz[i]) std::memcpy(zz[i], rec, nout * 8);
+    if (dd && dd[i]) std::memcpy(dd[i], rec + nout, nout * 8);
+    int fl[2];
+    std::memcpy(fl, rec + 2 * nout, 8);
+    if (fl[1]) return gfail(grp, SP_EARG, "evaluation list overflow");
+    if (diverged) diverged[i] = fl[0] ? 1 : 0;
+    c->have_flow = true;
+    if (grp->h_fd[j].euler && !fl[0]) {
+      c->cutoff = new_cutoffs[i];
+      c->have_flow = false;
+    }
+  }
+  return SP_OK;
+}
+
+int sp_group_measure(sp_group *grp, sp_ctx *const *ctxs, int n, double *const *zz, double *const *dd) {
+  if (!grp) return SP_EARG;
+  std::vector<size_t> off(n > 0 ? n : 1);
+  int rc = group_prepare(grp, ctxs, n, off.data());
+  if (rc) return rc;
+  SP_GHIP(hipMemcpyAsync(grp->d_fd, grp->h_fd.data(), sizeof(FlowDev) * n, hipMemcpyHostToDevice, grp->stream));
+  SP_GHIP(launch_measure(grp->stream, grp->d_fd, grp->h_fd.data(), n));
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total = std::max(total, off[i] + 2 * (size_t)ctxs[i]->fd.B * ctxs[i]->fd.Rb);
+  SP_GHIP(hipMemcpyAsync(grp->h_out, grp->d_out, total * 8, hipMemcpyDeviceToHost, grp->stream));
+  SP_GHIP(hipStreamSynchronize(grp->stream));
+  for (int i = 0; i < n; ++i) {
+    const size_t nout = (size_t)ctxs[i]->fd.B * ctxs[i]->fd.Rb;
+    if (zz && zz[i]) std::memcpy(zz[i], grp->h_out + off[i], nout * 8);
+    if (dd && dd[i]) std::memcpy(dd[i], grp->h_out + off[i] + nout, nout * 8);
+  }
   return SP_OK;
 }
 

@@ -674,16 +674,20 @@ def new_lattice_model(uc: LatticeUnitCell, model: SpinModelUnitCell, lattice_ran
         overlaps.append((np.array(r1, dtype=np.int32), np.array(r2, dtype=np.int32),
                          np.array(p1, dtype=np.int8).reshape(-1, 3), np.array(p2, dtype=np.int8).reshape(-1, 3)))
 
+    return Lattice(size=size, sites=sites, bravais=list(uc.lattice_vectors), basis=list(uc.basis_sites),
+                   basis_ids=basis_ids, ranges=ranges, sym_rid=sym_rid, sym_perm=sym_perm,
+                   overlaps=overlaps, couplings=model_couplings(sites[:size], model))
+
+
+def model_couplings(rep_sites: Sequence[Site], model: SpinModelUnitCell) -> List[Tuple[int, np.ndarray]]:
+    """Couplings (origin -> representative rid) in the order of the reference's
+    ``SpinModel::interactions`` (``src/LatticeModelFactory.cpp:957-985``)."""
     couplings = []
-    for r in range(size):
-        s = sites[r]
+    for r, s in enumerate(rep_sites):
         for inter in model.interactions:
             conn = inter.is_connecting_sites((0, 0, 0, 0), s)
             if conn == 1:
                 couplings.append((r, inter.strength.copy()))
             elif conn == -1:
                 couplings.append((r, inter.strength.T.copy()))
-
-    return Lattice(size=size, sites=sites, bravais=list(uc.lattice_vectors), basis=list(uc.basis_sites),
-                   basis_ids=basis_ids, ranges=ranges, sym_rid=sym_rid, sym_perm=sym_perm,
-                   overlaps=overlaps, couplings=couplings)
+    return couplings
