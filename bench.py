@@ -1,26 +1,41 @@
 #!/usr/bin/env python3
-"""Benchmark: full FP64 pf-FRG flow on one MI355X.
+"""Benchmark of the pf-FRG flow engine on one MI355X.
 
-Workload (default): SU(2) simple cubic J1 = 1, J2 = 0.3, lattice range 6
-(L = 23 representative sites, 377 sites in range), 64 log-spaced positive
-frequencies in [0.002, 100] (N_f = 133,120 frequency triples), cutoff 100 ->
-0.005 with step 0.985 (656 cutoffs, 655 derivative evaluations).  Synthetic in
-the sense that it is generated from the task parameters; the initial vertex is
-the model's couplings, exactly as the reference initialises a flow.
+Mode "flow" (default).  Workload: SU(2) simple cubic J1 = 1, J2 = 0.3, lattice
+range 6 (L = 23 representative sites, 377 sites in range), 64 log-spaced
+positive frequencies in [0.002, 100] (N_f = 133,120 frequency triples), cutoff
+100 -> 0.005 with step 0.985 (656 cutoffs, 655 derivative evaluations).
+Synthetic in the sense that it is generated from the task parameters; the
+initial vertex is the model's couplings, exactly as the reference initialises a
+flow.
 
 One step = one iteration of the reference's flow loop (src/SpinParser.cpp:141-160):
 computeStep (dSigma + dGamma), correlation measurement, divergence check,
 finalizeStep (in-place Euler).  The timed steps walk the cutoff schedule from the
 start; K = 655 is the full flow and yields the full-schedule average of
-dGamma/dLambda evaluations per second.  Warm-up steps run first on the same
-context, which is then reset to the initial condition.
+dGamma/dLambda evaluations per second (``value``, FP64 form).  Beside it:
 
-Roofline: the dGamma kernel is timed with HIP events on the context's stream;
-achieved = algorithmic FLOP of the timed evaluations (exact integrand-call
-counts x per-call work model) / summed kernel time, against the MI355X FP64
-peak.  The CPU baseline times the reference-arithmetic (float) CPU oracle with
-OpenMP over items on a bounded sample of the same workload and extrapolates by
-integrand-call counts over the schedule.
+* ``mixed``: the same flow in the mixed-precision form (FP32 frequency
+  integrals with compensated sums; FP64 state, Euler, self-energy and
+  correlations), timed the same way, with the max-relative error against the
+  FP64 flow of every output: correlations (every record), final state, and the
+  derivative on a common state;
+* ``fixed_cutoff``: evals/s at the first, middle and last cutoff with the
+  algorithmic work F(cutoff) of one evaluation;
+* ``roofline``: the dGamma kernel timed with HIP events on the context's
+  stream; achieved = algorithmic FLOP of the timed evaluations (exact
+  integrand-call counts x per-call work model) / summed kernel time, against
+  the MI355X FP64 peak; ``traffic`` = HBM bytes per launch from rocprofv3
+  FETCH_SIZE / WRITE_SIZE (table below);
+* ``cpu_baseline``: the reference-arithmetic CPU oracle with OpenMP over items
+  on a bounded sample of the same workload, extrapolated by integrand-call
+  counts over the schedule.
+
+Mode "sweep": the served J2/J1 phase-diagram sweep (64 SU(2) cubic flows, J2 ~
+U[0,1], range ~ U{3..7}, N_w ~ U{32,48,64}, Poisson arrivals, mixed precision,
+admission by device footprint) against the same flows run one at a time in
+FP64; flows/hour, p50/p99 latency and the correlation error of every served flow
+against its FP64 run.
 """
 import argparse
 import json
@@ -33,13 +48,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from spinparser_amd.capi import KERNEL_DGAMMA, FlowContext  # noqa: E402
+from spinparser_amd.capi import (KERNEL_DGAMMA, SP_PRECISION_FP64, SP_PRECISION_MIXED,  # noqa: E402
+                                 FlowContext, footprint_bytes)
 from spinparser_amd.problem import su2_problem  # noqa: E402
 from spinparser_amd.task import make_task_xml, parse_task  # noqa: E402
-from spinparser_amd.workload import compulsory_bytes, evaluation_calls, evaluation_flops  # noqa: E402
+from spinparser_amd.workload import compulsory_bytes, evaluation_flops  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+HBM_BYTES = 288 * 1024 ** 3
 
 WORKLOADS = {
     "cubic-j1j2-r6-nw64": dict(lattice="cubic", lattice_range=6, model="cubic-j1j2heisenberg",
@@ -50,10 +67,30 @@ WORKLOADS = {
                            cutoff_step=0.98),
 }
 
+# HBM bytes per k_dgamma launch over the full flow, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over the 655 dispatches
+# of the full flow).  Each counter is multiplied by the ratio of known bytes to
+# counter reading for a plain streaming kernel of known size (k_euler reads
+# Gamma and dGamma, writes Gamma: FETCH_SIZE reads half the known bytes, scale
+# 2.00; WRITE_SIZE reads them exactly, scale 1.00).  The profile summaries
+# under profiles/ hold the runs and the commands.
+TRAFFIC_KIB = {
+    ("cubic-j1j2-r6-nw64", "f64"): {"fetch_kib": 1454910.0, "write_kib": 3975740.0,
+                                    "fetch_scale": 2.00, "write_scale": 1.00},
+}
+
 
 def build_problem(name):
     xml = make_task_xml(symmetry="SU2", **WORKLOADS[name])
     return su2_problem(parse_task(xml, is_text=True))
+
+
+def max_rel(a, b):
+    """Normwise relative error max|a-b| / max|b|."""
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    den = float(np.abs(b).max())
+    num = float(np.abs(a - b).max())
+    return num / den if den > 0 else num
 
 
 def cpu_baseline(prob, target_s=15.0):
@@ -99,60 +136,105 @@ def cpu_baseline(prob, target_s=15.0):
     }
 
 
-def run_step(ctx, prob, k):
+def run_step(ctx, prob, k, records=None):
     ctx.compute_step()
-    ctx.measure_correlation()
+    zz, dd = ctx.measure_correlation()
+    if records is not None:
+        records.append((zz, dd))
     if ctx.is_diverged():
         raise RuntimeError("flow diverged at cutoff index %d" % k)
     ctx.finalize_step(float(prob.cutoffs[k + 1]))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=655)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cubic-j1j2-r6-nw64", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this engine runs one flow per GPU and is measured on one GPU only; "
-              "--gpus %d is not supported (the multi-GPU sharding is a design, not a build)" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
-
-    prob = build_problem(args.workload)
+def flow_leg(prob, precision, steps, warmup, label):
+    """Time `steps` iterations of the flow loop from the first cutoff."""
     nsteps_flow = len(prob.cutoffs) - 1
-    ctx = FlowContext(prob)
+    ctx = FlowContext(prob, precision)
     init = prob.initial_state(np.float64)
-
     ctx.set_state(float(prob.cutoffs[0]), *init)
-    for w in range(args.warmup):
+    for w in range(warmup):
         run_step(ctx, prob, w % nsteps_flow)
     ctx.set_state(float(prob.cutoffs[0]), *init)
     ctx.synchronize()
-
     ctx.profile(True)
     flops = 0.0
+    records = []
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         k = i % nsteps_flow
         if k == 0 and i > 0:
             ctx.set_state(float(prob.cutoffs[0]), *init)
-        run_step(ctx, prob, k)
+            records = []
+        run_step(ctx, prob, k, records)
         flops += evaluation_flops(prob, prob.cutoffs[k])
         if (i + 1) % 100 == 0:
-            print("# step %d / %d  cutoff %.5g  %.1f s" % (i + 1, args.steps, prob.cutoffs[k],
-                                                          time.perf_counter() - t0), file=sys.stderr, flush=True)
+            print("# %s step %d / %d  cutoff %.5g  %.1f s" % (label, i + 1, steps, prob.cutoffs[k],
+                                                             time.perf_counter() - t0), file=sys.stderr, flush=True)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     dg_ms, dg_n = ctx.profile_get(KERNEL_DGAMMA)
     ctx.profile(False)
+    final = ctx.get_state() if steps % nsteps_flow == 0 else None
     ctx.close()
+    return dict(elapsed=elapsed, dg_ms=dg_ms, dg_n=dg_n, flops=flops, records=records, final=final)
 
-    achieved = flops / (dg_ms * 1e-3) / 1e12
+
+def fixed_cutoff(prob, precision, reps=5):
+    """evals/s at the first, middle and last evaluated cutoff.  The work of an
+    evaluation depends on the cutoff only (integration windows), so the state
+    is the initial condition placed at that cutoff."""
+    ncut = len(prob.cutoffs)
+    init = prob.initial_state(np.float64)
+    out = []
+    ctx = FlowContext(prob, precision)
+    for k in (0, (ncut - 2) // 2, ncut - 2):
+        ctx.set_state(float(prob.cutoffs[k]), *init)
+        ctx.compute_step()
+        ctx.synchronize()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.compute_step()
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        dg_ms, dg_n = ctx.profile_get(KERNEL_DGAMMA)
+        ctx.profile(False)
+        f = evaluation_flops(prob, prob.cutoffs[k])
+        kern = dg_ms * 1e-3 / max(dg_n, 1)
+        out.append({"cutoff": float(prob.cutoffs[k]), "gflop_per_eval": f / 1e9, "evals_per_s": 1.0 / wall,
+                    "k_dgamma_ms": kern * 1e3, "tflops_k_dgamma": f / kern / 1e12})
+    ctx.close()
+    return out
+
+
+def derivative_errors(prob, state):
+    """Mixed vs FP64 derivative on one common state."""
+    flows = []
+    for prec in (SP_PRECISION_FP64, SP_PRECISION_MIXED):
+        ctx = FlowContext(prob, prec)
+        ctx.set_state(*state)
+        ctx.compute_step()
+        flows.append(ctx.get_flow())
+        ctx.close()
+    names = ("dSigma", "dGamma_spin", "dGamma_density")
+    return {n: max_rel(a, b) for n, a, b in zip(names, flows[1], flows[0])}
+
+
+def traffic_entry(workload, dtype):
+    t = TRAFFIC_KIB.get((workload, dtype))
+    if t is None:
+        return None
+    return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
+
+
+def mode_flow(args):
+    prob = build_problem(args.workload)
+    nsteps_flow = len(prob.cutoffs) - 1
+    r64 = flow_leg(prob, SP_PRECISION_FP64, args.steps, args.warmup, "fp64")
+    achieved = r64["flops"] / (r64["dg_ms"] * 1e-3) / 1e12
+    elapsed = r64["elapsed"]
     hbm_gbs = compulsory_bytes(prob) * args.steps / elapsed / 1e9
+    traffic = traffic_entry(args.workload, "f64")
     out = {
         "metric": "dGamma/dLambda evals/s (full-schedule average, FP64 flow)",
         "value": args.steps / elapsed,
@@ -171,14 +253,136 @@ def main():
                    "step": "computeStep + correlation + isDiverged + finalizeStep"},
         "hbm_gbs_compulsory": hbm_gbs,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_dgamma", "launches": dg_n, "avg_launch_ms": dg_ms / max(dg_n, 1),
-                     "algorithmic_tflop_total": flops / 1e12},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "k_dgamma", "launches": r64["dg_n"], "avg_launch_ms": r64["dg_ms"] / max(r64["dg_n"], 1),
+                     "algorithmic_tflop_total": r64["flops"] / 1e12,
+                     "peak_note": "FP64 vector FMA peak (no MFMA: gathers and sparse site sums)"},
         "cpu_baseline": None,
     }
+    if traffic is not None:
+        out["roofline"]["traffic_gbs"] = traffic / (out["roofline"]["avg_launch_ms"] * 1e-3) / 1e9
+    if not args.no_fixed:
+        out["fixed_cutoff"] = fixed_cutoff(prob, SP_PRECISION_FP64)
+    if not args.no_mixed:
+        r32 = flow_leg(prob, SP_PRECISION_MIXED, args.steps, 0, "mixed")
+        mixed = {"evals_per_s": args.steps / r32["elapsed"], "ms_per_step": r32["elapsed"] * 1e3 / args.steps,
+                 "k_dgamma_avg_ms": r32["dg_ms"] / max(r32["dg_n"], 1),
+                 "speedup_vs_fp64": elapsed / r32["elapsed"]}
+        if r64["records"] and len(r64["records"]) == len(r32["records"]):
+            mixed["max_rel_err_corr_zz"] = max(max_rel(a[0], b[0]) for a, b in zip(r32["records"], r64["records"]))
+            mixed["max_rel_err_corr_dd"] = max(max_rel(a[1], b[1]) for a, b in zip(r32["records"], r64["records"]))
+            mixed["records_compared"] = len(r64["records"])
+        if r64["final"] is not None and r32["final"] is not None:
+            names = ("Sigma", "Gamma_spin", "Gamma_density")
+            mixed["max_rel_err_final_state"] = {n: max_rel(a, b) for n, a, b in
+                                                zip(names, r32["final"][1:], r64["final"][1:])}
+            mixed["max_rel_err_derivative_final_state"] = derivative_errors(prob, r64["final"])
+        init = prob.initial_state(np.float64)
+        mid = float(prob.cutoffs[(nsteps_flow - 1) // 2])
+        mixed["max_rel_err_derivative_initial_state_mid_cutoff"] = derivative_errors(prob, (mid, *init))
+        out["mixed"] = mixed
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
     print(json.dumps(out))
+
+
+def mode_sweep(args):
+    from spinparser_amd.serve import SweepServer, run_alone, sweep_requests
+    t0 = time.perf_counter()
+    reqs = sweep_requests(args.flows, seed=args.seed, rate_per_s=args.rate)
+    t_build = time.perf_counter() - t0
+    print("# %d requests built in %.1f s" % (len(reqs), t_build), file=sys.stderr, flush=True)
+    fps = [footprint_bytes(r.problem, SP_PRECISION_MIXED) for r in reqs]
+    budget = min(HBM_BYTES - (16 << 30), int(args.budget_gb * (1 << 30)))
+    srv = SweepServer(budget_bytes=budget, max_flows=args.flows, precision=SP_PRECISION_MIXED)
+    t0 = time.perf_counter()
+    served = srv.run(reqs, log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    t_served = time.perf_counter() - t0
+    srv.close()
+    lat = np.array([r.latency for r in served])
+    out = {
+        "metric": "served sweep flows/hour (mixed precision, continuous batching)",
+        "value": len(served) / t_served * 3600.0,
+        "unit": "flows/hour",
+        "n_gpus": 1,
+        "steps": int(sum(r.steps for r in served)),
+        "warmup": 0,
+        "ms_per_step": t_served * 1e3 / max(1, sum(r.steps for r in served)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (cubic J1-J2 flows drawn from PCG64 seed %d)" % args.seed,
+        "config": {"workload": "cubic-j1j2-sweep-%d" % args.flows, "seed": args.seed, "rate_per_s": args.rate,
+                   "ranges": [3, 4, 5, 6, 7], "nws": [32, 48, 64], "budget_bytes": budget,
+                   "footprint_total_bytes": int(sum(fps)), "footprint_max_bytes": int(max(fps))},
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "sweep_wall_s": t_served,
+        "peak_flows_resident": srv.peak_flows,
+        "peak_bytes_resident": srv.peak_bytes,
+        "mean_batch": float(np.mean(srv.batch_sizes)),
+        "diverged": int(sum(r.diverged for r in served)),
+    }
+    if not args.no_alone:
+        t_alone = 0.0
+        worst_zz = worst_dd = 0.0
+        worst_zz_div = worst_dd_div = 0.0
+        same_end = 0
+        for r, s in zip(reqs, served):
+            a = run_alone(r, SP_PRECISION_FP64)
+            t_alone += a.latency
+            same_end += int(len(a.zz) == len(s.zz) and a.diverged == s.diverged)
+            # records up to the last one before either run diverged: a flow
+            # that diverges has its final records dominated by the blow-up,
+            # where relative comparisons carry no information
+            n = min(len(a.zz), len(s.zz))
+            if a.diverged or s.diverged:
+                n_ok = max(0, n - 10)
+                if n_ok:
+                    worst_zz_div = max(worst_zz_div, max(max_rel(x, y) for x, y in zip(s.zz[:n_ok], a.zz[:n_ok])))
+                    worst_dd_div = max(worst_dd_div, max(max_rel(x, y) for x, y in zip(s.dd[:n_ok], a.dd[:n_ok])))
+            else:
+                worst_zz = max(worst_zz, max(max_rel(x, y) for x, y in zip(s.zz[:n], a.zz[:n])))
+                worst_dd = max(worst_dd, max(max_rel(x, y) for x, y in zip(s.dd[:n], a.dd[:n])))
+            print("# alone FP64 flow %d %s: %.2f s, %d records, diverged %s (served: %d, %s)" % (
+                r.ident, r.params, a.latency, len(a.zz), a.diverged, len(s.zz), s.diverged),
+                file=sys.stderr, flush=True)
+        out["fp64_one_at_a_time"] = {"wall_s": t_alone, "flows_per_hour": len(reqs) / t_alone * 3600.0}
+        out["speedup_vs_fp64_one_at_a_time"] = out["value"] / out["fp64_one_at_a_time"]["flows_per_hour"]
+        out["max_rel_err_corr_vs_fp64"] = {"zz": worst_zz, "dd": worst_dd,
+                                           "flows": len(reqs) - int(sum(r.diverged for r in served))}
+        out["max_rel_err_corr_vs_fp64_diverging_flows_before_last_10"] = {"zz": worst_zz_div, "dd": worst_dd_div}
+        out["flows_same_end_as_fp64"] = same_end
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=655)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="flow", choices=("flow", "sweep"))
+    ap.add_argument("--workload", default="cubic-j1j2-r6-nw64", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-fixed", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--flows", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--rate", type=float, default=1.0, help="Poisson arrival rate of the sweep (flows/s)")
+    ap.add_argument("--budget-gb", type=float, default=256.0)
+    ap.add_argument("--no-alone", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU and is measured on one GPU only; "
+              "--gpus %d is not supported (the multi-GPU sharding is a design, not a build)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if args.mode == "flow":
+        mode_flow(args)
+    else:
+        mode_sweep(args)
 
 
 if __name__ == "__main__":

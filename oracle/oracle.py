@@ -142,3 +142,109 @@ class Oracle:
         lib().or_integrate_test_f32(ctypes.c_int(len(mesh)), _ptr(mesh, _f32p), ctypes.c_int(kind),
                                     ctypes.c_float(a), ctypes.c_float(b), _ptr(out, _f32p))
         return out
+
+
+# ---------------------------------------------------------------------------
+# TRI core (tri_oracle.cpp)
+# ---------------------------------------------------------------------------
+TRI_LIB_PATH = os.path.join(HERE, "liboracle_tri.so")
+TRI_GROUPS = ("TRICorDD", "TRICorXX", "TRICorXY", "TRICorXZ", "TRICorYX", "TRICorYY", "TRICorYZ", "TRICorZX",
+              "TRICorZY", "TRICorZZ")
+
+
+class OrTriProblem(ctypes.Structure):
+    _fields_ = [("nw", ctypes.c_int), ("mesh", _f32p), ("L", ctypes.c_int),
+                ("site_rid", _i32p), ("site_perm", _i32p), ("inv_rid", _i32p), ("inv_perm", _i32p),
+                ("ovl_off", _i32p), ("ovl_rid1", _i32p), ("ovl_rid2", _i32p), ("ovl_perm1", _i32p),
+                ("ovl_perm2", _i32p), ("R", ctypes.c_int), ("range_fwd", _i32p), ("range_inv", _i32p),
+                ("B", ctypes.c_int), ("Rb", ctypes.c_int), ("corr_rid", _i32p), ("corr_perm", _i32p)]
+
+
+_tri_lib = None
+
+
+def tri_lib():
+    global _tri_lib
+    if _tri_lib is None:
+        if not os.path.exists(TRI_LIB_PATH):
+            build()
+        _tri_lib = ctypes.CDLL(TRI_LIB_PATH)
+    return _tri_lib
+
+
+class TriOracle:
+    """CPU TRI flow (16 spin components) on a ``TRIProblem``."""
+
+    def __init__(self, prob):
+        self.prob = prob
+        arrs = (prob.mesh.astype(np.float32), prob.site_rid, prob.site_perm, prob.inv_rid, prob.inv_perm,
+                prob.ovl_off, prob.ovl_rid1, prob.ovl_rid2, prob.ovl_perm1, prob.ovl_perm2, prob.range_fwd,
+                prob.range_inv, prob.corr_rid, prob.corr_perm)
+        self._keep = [np.ascontiguousarray(a) if a.dtype == np.float32 else np.ascontiguousarray(a, dtype=np.int32)
+                      for a in arrs]
+        k = self._keep
+        self.c = OrTriProblem(prob.nw, _ptr(k[0], _f32p), prob.L, _ptr(k[1], _i32p), _ptr(k[2], _i32p),
+                              _ptr(k[3], _i32p), _ptr(k[4], _i32p), _ptr(k[5], _i32p), _ptr(k[6], _i32p),
+                              _ptr(k[7], _i32p), _ptr(k[8], _i32p), _ptr(k[9], _i32p), len(prob.range_fwd),
+                              _ptr(k[10], _i32p), _ptr(k[11], _i32p), prob.B, prob.Rb, _ptr(k[12], _i32p),
+                              _ptr(k[13], _i32p))
+        self.lib = tri_lib()
+
+    def _t(self, dtype):
+        return (_f64p, ctypes.c_double, "f64") if dtype == np.float64 else (_f32p, ctypes.c_float, "f32")
+
+    def compute_step(self, cutoff, v2, v4, dtype=np.float64, nthreads=None):
+        """(dv2, dv4, calls[S,T,U]) at one cutoff."""
+        nthreads = nthreads or os.cpu_count() or 1
+        p = self.prob
+        t, ct, sfx = self._t(dtype)
+        v2, v4 = (np.ascontiguousarray(x, dtype=dtype) for x in (v2, v4))
+        dv2 = np.zeros(p.nw, dtype=dtype)
+        dv4 = np.zeros(p.vertex_size, dtype=dtype)
+        calls = np.zeros(3, dtype=np.int64)
+        getattr(self.lib, "or_tri_compute_step_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t),
+                                                         _ptr(v4, t), _ptr(dv2, t), _ptr(dv4, t),
+                                                         ctypes.c_int(nthreads), _ptr(calls, _lp))
+        return dv2, dv4, calls
+
+    def items(self, cutoff, v2, dv2, v4, it0, it1, dtype=np.float64, nthreads=None):
+        nthreads = nthreads or os.cpu_count() or 1
+        t, ct, sfx = self._t(dtype)
+        v2, dv2, v4 = (np.ascontiguousarray(x, dtype=dtype) for x in (v2, dv2, v4))
+        dv4 = np.zeros(self.prob.vertex_size, dtype=dtype)
+        calls = np.zeros(3, dtype=np.int64)
+        getattr(self.lib, "or_tri_items_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t), _ptr(dv2, t),
+                                                  _ptr(v4, t), ctypes.c_int(it0), ctypes.c_int(it1), _ptr(dv4, t),
+                                                  ctypes.c_int(nthreads), _ptr(calls, _lp))
+        return dv4, calls
+
+    def correlation(self, cutoff, v2, v4, dtype=np.float64):
+        """dict group -> (B, Rb) array of the 10 TRI correlation outputs."""
+        p = self.prob
+        t, ct, sfx = self._t(dtype)
+        v2, v4 = (np.ascontiguousarray(x, dtype=dtype) for x in (v2, v4))
+        out = np.zeros(10 * p.B * p.Rb, dtype=dtype)
+        getattr(self.lib, "or_tri_correlation_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t),
+                                                        _ptr(v4, t), _ptr(out, t))
+        out = out.reshape(10, p.B, p.Rb)
+        return {g: out[i] for i, g in enumerate(TRI_GROUPS)}
+
+    def run_flow(self, dtype=np.float32, nthreads=None, ncut=None):
+        """Full flow; returns (records[n, 10, B, Rb], n_records, (v2, v4))."""
+        nthreads = nthreads or os.cpu_count() or 1
+        p = self.prob
+        cut = np.ascontiguousarray(p.cutoffs if ncut is None else p.cutoffs[:ncut], dtype=np.float32)
+        v2, v4 = p.initial_state(dtype)
+        rec = np.zeros((len(cut), 10, p.B, p.Rb), dtype=dtype)
+        t, ct, sfx = self._t(dtype)
+        n = getattr(self.lib, "or_tri_run_flow_" + sfx)(ctypes.byref(self.c), ctypes.c_int(len(cut)),
+                                                         _ptr(cut, _f32p), _ptr(v2, t), _ptr(v4, t), _ptr(rec, t),
+                                                         ctypes.c_int(nthreads))
+        return rec, n, (v2, v4)
+
+    @staticmethod
+    def table(kind):
+        """Dense (16, 16, 16) coefficient table: 0 S, 1 U, 2 chalice, 3 inverse chalice, 4 lattice bubble."""
+        out = np.zeros(4096)
+        n = tri_lib().or_tri_table(ctypes.c_int(kind), _ptr(out, _f64p))
+        return out.reshape(16, 16, 16), n

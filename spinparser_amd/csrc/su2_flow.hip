@@ -71,6 +71,9 @@ namespace {
 #ifndef SPFRG_MIN_BLOCKS
 #define SPFRG_MIN_BLOCKS 4
 #endif
+#ifndef SPFRG_MIN_BLOCKS_MIXED
+#define SPFRG_MIN_BLOCKS_MIXED 1
+#endif
 constexpr int kThreads = SPFRG_THREADS;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoPi = 2.0 * kPi;
@@ -496,7 +499,7 @@ struct PairBuf {
 };
 
 template <bool MIXED>
-__global__ __launch_bounds__(kThreads, SPFRG_MIN_BLOCKS) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,
+__global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOCKS) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,
                                                      long long per_xcd) {
   using V = typename std::conditional<MIXED, float, double>::type;
   using V2 = typename std::conditional<MIXED, float2, double2>::type;
@@ -620,7 +623,11 @@ __global__ __launch_bounds__(kThreads, SPFRG_MIN_BLOCKS) void k_dgamma(const Flo
         }
         const Interp i0 = interp_of(s_mesh, nw, x0);
         const Interp i1 = interp_of(s_mesh, nw, x1);
-        for (int q = 0; q < nbufs; ++q) {
+        // fully unrolled with a guard so that pb[] stays in registers (a
+        // dynamically indexed local array would live in scratch memory)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (q >= nbufs) break;
           const PairBuf &b = pb[q];
           const Desc4 d = assemble(b.chan, xo, b.swap ? i1 : i0, b.swap ? i0 : i1, nw, rowbytes);
           if (b.k < 4) {

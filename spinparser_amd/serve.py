@@ -89,7 +89,8 @@ class SweepServer:
         ctx.set_state(float(req.problem.cutoffs[0]), v2, v4s, v4d)
         return _Active(req, ctx, fp, now)
 
-    def run(self, requests: List[FlowRequest], log: Optional[Callable[[str], None]] = None) -> List[FlowResult]:
+    def run(self, requests: List[FlowRequest], log: Optional[Callable[[str], None]] = None,
+            log_every: int = 50) -> List[FlowResult]:
         pending = sorted(requests, key=lambda r: (r.arrival, r.ident))
         for r in pending:
             if footprint_bytes(r.problem, self.precision) > self.budget:
@@ -111,6 +112,9 @@ class SweepServer:
                 self.sleep(max(0.0, pending[0].arrival - now))
                 continue
             self.batch_sizes.append(len(active))
+            if log and log_every > 0 and len(self.batch_sizes) % log_every == 0:
+                log("iteration %d: %d active, %d pending, %d done, %.1f GB resident, t=%.1f s" % (
+                    len(self.batch_sizes), len(active), len(pending), len(done), used / 1e9, now))
             new_cut = [float(a.prob.cutoffs[a.k + 1]) for a in active]
             outs, div = self.group.step([a.ctx for a in active], new_cut)
             finishing: List[_Active] = []

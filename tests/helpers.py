@@ -64,3 +64,25 @@ def max_rel(a, b):
     if den == 0.0:
         return num
     return num / den
+
+
+def tri_ref3_problem():
+    """TRI honeycomb range 3, Heisenberg-Kitaev-Gamma j=0.2 k=1 g=-0.1 (test/scripted/test_reference3.sh:17-60)."""
+    from spinparser_amd.problem import tri_problem
+    xml = manual_task_xml("honeycomb", 3, "honeycomb-kitaev-gamma", {"j": "0.2", "k": "1.0", "g": "-0.1"},
+                          symmetry="TRI")
+    return tri_problem(parse_task(xml, True), GOLDEN_RES)
+
+
+def tri_problem_of(lattice, lattice_range, model, couplings, nw, fmin, fmax, cmax, cmin, cstep):
+    from spinparser_amd.problem import tri_problem
+    xml = make_task_xml(lattice=lattice, lattice_range=lattice_range, model=model, symmetry="TRI",
+                        couplings=couplings, freq_min=fmin, freq_max=fmax, freq_count=nw, cutoff_max=cmax,
+                        cutoff_min=cmin, cutoff_step=cstep)
+    return tri_problem(parse_task(xml, is_text=True))
+
+
+def cfg4_problem(nw=48, lattice_range=5):
+    """Kagome J=1 with out-of-plane DM D=0.2, range 5, log mesh [0.005, 50], cutoff 50 -> 0.01 x0.98."""
+    return tri_problem_of("kagome", lattice_range, "kagome-DM", {"j": 1.0, "d": 0.2}, nw, 0.005, 50.0, 50.0, 0.01,
+                          0.98)

@@ -1,0 +1,92 @@
+"""TRI oracle pins (CPU).
+
+* the reference's TRI golden flow (test/scripted/assets/test_reference3.ref:
+  honeycomb Heisenberg-Kitaev-Gamma, range 3, 14 frequencies, 34 cutoffs, all 10
+  TRICor* groups; the reference's own tolerance is 1e-5 absolute,
+  test/scripted/assets/test_eval.py:14,28-29) -- float instantiation;
+* TRI on a Heisenberg model equals the SU(2) core (test/scripted/test_reference1.sh:71-78):
+  XX = YY = ZZ = SU2CorZZ and DD = SU2CorDD;
+* structure of the Pauli-derived coefficient tables: term counts, and the
+  density-density particle-particle row, whose form is fixed by sigma^a sigma^a = 1.
+"""
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, manual_task_xml, tri_ref3_problem
+from oracle import TRI_GROUPS, Oracle, TriOracle
+
+
+@pytest.fixture(scope="module")
+def ref3_flow():
+    p = tri_ref3_problem()
+    rec, n, _ = TriOracle(p).run_flow(dtype=np.float32, nthreads=8)
+    return p, rec, n
+
+
+def test_tri_golden_flow(ref3_flow):
+    p, rec, n = ref3_flow
+    g = np.load(GOLDEN + "/reference3_tri.npz")
+    assert n == len(p.cutoffs) == 34
+    worst = 0.0
+    for k in range(n):
+        for gi, grp in enumerate(TRI_GROUPS):
+            assert abs(g["%s/data/measurement_%d/cutoff" % (grp, k)][0] - p.cutoffs[k]) == 0.0
+            ref = g["%s/data/measurement_%d/data" % (grp, k)]
+            worst = max(worst, float(np.abs(rec[k, gi] - ref).max()))
+    # float arithmetic of a different operation order: stays at rounding level
+    assert worst < 1e-6, worst
+
+
+def test_tri_golden_site_order(ref3_flow):
+    p, _, _ = ref3_flow
+    g = np.load(GOLDEN + "/reference3_tri.npz")
+    sites = g["TRICorXX/meta/sites"]
+    lat = p.lattice
+    for b in range(p.B):
+        for k, j in enumerate(lat.ranges[b]):
+            assert np.allclose(lat.position(j), sites[b, k], atol=1e-5)
+
+
+def test_tri_heisenberg_equals_su2():
+    from spinparser_amd.problem import su2_problem, tri_problem
+    from spinparser_amd.task import parse_task
+    from helpers import GOLDEN_RES
+    xs = manual_task_xml("honeycomb", 2, "honeycomb-heisenberg", {"j": "1.0"}, cmin="1.0")
+    xt = manual_task_xml("honeycomb", 2, "honeycomb-heisenberg", {"j": "1.0"}, symmetry="TRI", cmin="1.0")
+    ps = su2_problem(parse_task(xs, True), GOLDEN_RES)
+    pt = tri_problem(parse_task(xt, True), GOLDEN_RES)
+    zz, dd, n, _ = Oracle(ps).run_flow(dtype=np.float64, nthreads=8)
+    rec, nt, _ = TriOracle(pt).run_flow(dtype=np.float64, nthreads=8)
+    assert n == nt and n > 5
+    scale = np.abs(zz[:n]).max()
+    for gi, grp in enumerate(TRI_GROUPS):
+        if grp == "TRICorDD":
+            assert np.abs(rec[:n, gi] - dd[:n]).max() < 1e-12 * max(1.0, np.abs(dd[:n]).max())
+        elif grp[-1] == grp[-2]:
+            assert np.abs(rec[:n, gi] - zz[:n]).max() < 1e-12 * scale, grp
+        else:
+            assert np.abs(rec[:n, gi]).max() < 1e-12 * scale, grp
+
+
+def test_tri_tables_structure():
+    counts = [TriOracle.table(k)[1] for k in range(5)]
+    assert counts == [256, 256, 256, 256, 64]
+    S, _ = TriOracle.table(0)
+    # particle-particle density-density row: sum over components c of
+    # zeta-weighted products A[c] B[c] (sigma^a sigma^a = 1 on both lines; the
+    # i of mixed components squares to -1)
+    row = S[15]
+    for a in range(16):
+        for b in range(16):
+            if a != b:
+                assert row[a, b] == 0.0
+            else:
+                mu, nu = divmod(a, 4)
+                expect = (-1.0 if (mu == 3) != (nu == 3) else 1.0)
+                assert row[a, a] == expect
+    # every coefficient is +-1 (+-2 in the lattice bubble)
+    for k in range(4):
+        t, _ = TriOracle.table(k)
+        assert set(np.unique(np.abs(t))) <= {0.0, 1.0}
+    t, _ = TriOracle.table(4)
+    assert set(np.unique(np.abs(t))) == {0.0, 2.0}
