@@ -140,6 +140,72 @@ int sp_group_step(sp_group *group, sp_ctx *const *ctxs, int n, const double *new
 int sp_group_measure(sp_group *group, sp_ctx *const *ctxs, int n, double *const *zz, double *const *dd);
 const char *sp_group_last_error(const sp_group *group);
 
+/* ------------------------------------------------------------------------
+ * TRI core: general time-reversal-invariant two-spin couplings, 16 spin
+ * components per vertex entry (src/TRI/TRIFrgCore.cpp).  FP64.
+ *
+ * Vertex layout (src/TRI/TRIVertexTwoParticle.hpp:62-76,657-666):
+ *   Gamma[((s(s+1)/2 + u) * Nw + t) * 16 L + (4 s1 + s2) L + rid], s >= u,
+ *   spin components s1, s2 in (x, y, z, d) = (0, 1, 2, 3); Sigma[Nw].
+ * Spin permutations are 3 entries per site transformation, the image of the
+ * components x, y, z (LatticeSiteDescriptor::spinPermutation,
+ * src/Lattice.hpp:150-161; the overlap transforms transformedX/Y/Z 1/2,
+ * src/Lattice.hpp:38-75).
+ * ---------------------------------------------------------------------- */
+typedef struct sp_tri_problem {
+  int nw;                 /* positive Matsubara frequencies N_w               */
+  const float *mesh;      /* N_w ascending positive mesh points               */
+  int L;                  /* representative lattice sites                      */
+  const int *site_rid;    /* L: getSites()[j].rid                              */
+  const int *site_perm;   /* 3L: getSites()[j].spinPermutation                 */
+  const int *inv_rid;     /* L: getInvertedSites()[j].rid                      */
+  const int *inv_perm;    /* 3L: getInvertedSites()[j].spinPermutation         */
+  const int *ovl_off;     /* L+1: CSR offsets of getOverlap(rid)               */
+  const int *ovl_rid1;    /* O: overlap rid1                                   */
+  const int *ovl_rid2;    /* O: overlap rid2                                   */
+  const int *ovl_perm1;   /* 3O: transformedX1, Y1, Z1                         */
+  const int *ovl_perm2;   /* 3O: transformedX2, Y2, Z2                         */
+  int R;                  /* sites in range of the origin                      */
+  const int *range_fwd;   /* R: symmetryTransform(0, j)                        */
+  const int *range_inv;   /* R: symmetryTransform(j, 0)                        */
+  int B;                  /* basis sites                                       */
+  int Rb;                 /* sites in range of each basis site                 */
+  const int *corr_rid;    /* B*Rb: symmetryTransform(b, j) rid                 */
+  const int *corr_perm;   /* 3*B*Rb: its spin transformation of x, y, z        */
+} sp_tri_problem;
+
+typedef struct sp_tri_ctx sp_tri_ctx;
+
+/* Construction of TRIFrgCore + TRIEffectiveAction (src/TRI/TRIFrgCore.cpp:17-71). */
+int sp_tri_ctx_create(const sp_tri_problem *problem, sp_tri_ctx **out);
+/* TRIFrgCore::~TRIFrgCore (src/TRI/TRIFrgCore.cpp:73-77). */
+int sp_tri_ctx_destroy(sp_tri_ctx *ctx);
+/* State upload: the initial condition / readCheckpoint of TRIEffectiveAction
+ * (src/TRI/TRIEffectiveAction.hpp:38-64,159-205). */
+int sp_tri_set_state(sp_tri_ctx *ctx, double cutoff, const double *v2, const double *v4);
+/* State download (writeCheckpoint's read side, src/TRI/TRIEffectiveAction.hpp:79-150). */
+int sp_tri_get_state(sp_tri_ctx *ctx, double *cutoff, double *v2, double *v4);
+/* FrgCore::_flow of the last step (src/FrgCore.hpp:140). */
+int sp_tri_get_flow(sp_tri_ctx *ctx, double *dv2, double *dv4);
+/* TRIFrgCore::computeStep (src/TRI/TRIFrgCore.cpp:79-96,122-3001).  Asynchronous. */
+int sp_tri_compute_step(sp_tri_ctx *ctx);
+/* TRIFrgCore::finalizeStep (src/TRI/TRIFrgCore.cpp:98-120).  Asynchronous. */
+int sp_tri_finalize_step(sp_tri_ctx *ctx, double new_cutoff);
+/* TRIEffectiveAction::isDiverged on the flow (src/TRI/TRIEffectiveAction.hpp:212-230). */
+int sp_tri_is_diverged(sp_tri_ctx *ctx, int *diverged);
+/* TRIMeasurementCorrelation::_calculateCorrelation
+ * (src/TRI/TRIMeasurementCorrelation.cpp:147-316): 10*B*Rb values, the groups
+ * TRICorDD, XX, XY, XZ, YX, YY, YZ, ZX, ZY, ZZ in this order, each in the
+ * .obs site order.  Synchronous. */
+int sp_tri_measure_correlation(sp_tri_ctx *ctx, double *out);
+int sp_tri_synchronize(sp_tri_ctx *ctx);
+const char *sp_tri_last_error(const sp_tri_ctx *ctx);
+/* Kernel timing (kernel ids as for the SU(2) core). */
+int sp_tri_profile_enable(sp_tri_ctx *ctx, int enable);
+int sp_tri_profile_get(sp_tri_ctx *ctx, int kernel, double *total_ms, long *launches);
+/* Device bytes held by a TRI context. */
+int sp_tri_ctx_footprint(const sp_tri_problem *problem, long long *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,7 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB_DIR = os.path.join(PKG, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspfrg.so")
-SOURCES = [os.path.join(PKG, "csrc", "su2_flow.hip")]
+SOURCES = [os.path.join(PKG, "csrc", "su2_flow.hip"), os.path.join(PKG, "csrc", "tri_flow.hip")]
 
 
 def hipcc() -> str:
@@ -32,7 +32,7 @@ def build_lib(verbose: bool = False) -> str:
     tmp = LIB_PATH + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include"),
+           "-fconstexpr-steps=33554432", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include"),
            "-o", tmp] + SOURCES
     if verbose:
         print(" ".join(cmd))

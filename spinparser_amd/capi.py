@@ -83,6 +83,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.sp_group_measure.argtypes = [c_grp, _pp, ctypes.c_int, ctypes.POINTER(_f64p), ctypes.POINTER(_f64p)]
     lib.sp_group_last_error.argtypes = [c_grp]
     lib.sp_group_last_error.restype = ctypes.c_char_p
+    _declare_tri(lib)
     if path is None:
         _lib = lib
     return lib
@@ -251,3 +252,153 @@ class FlowGroup:
         dd = (_f64p * n)(*[_p(o[1], _f64p) for o in outs])
         self._check(self.lib.sp_group_measure(self.grp, ctxs, n, zz, dd), "sp_group_measure")
         return [(o[0].reshape(f.prob.B, f.prob.Rb), o[1].reshape(f.prob.B, f.prob.Rb)) for o, f in zip(outs, flows)]
+
+
+# ---------------------------------------------------------------------------
+# TRI core (sp_tri_*)
+# ---------------------------------------------------------------------------
+TRI_EXPORTED_SYMBOLS = (
+    "sp_tri_ctx_create", "sp_tri_ctx_destroy", "sp_tri_set_state", "sp_tri_get_state", "sp_tri_get_flow",
+    "sp_tri_compute_step", "sp_tri_finalize_step", "sp_tri_is_diverged", "sp_tri_measure_correlation",
+    "sp_tri_synchronize", "sp_tri_last_error", "sp_tri_profile_enable", "sp_tri_profile_get",
+    "sp_tri_ctx_footprint",
+)
+TRI_GROUPS = ("TRICorDD", "TRICorXX", "TRICorXY", "TRICorXZ", "TRICorYX", "TRICorYY", "TRICorYZ", "TRICorZX",
+              "TRICorZY", "TRICorZZ")
+
+
+def _declare_tri(lib):
+    c_ctx = ctypes.c_void_p
+    lib.sp_tri_ctx_create.argtypes = [ctypes.POINTER(SpTriProblem), ctypes.POINTER(c_ctx)]
+    lib.sp_tri_ctx_destroy.argtypes = [c_ctx]
+    lib.sp_tri_set_state.argtypes = [c_ctx, ctypes.c_double, _f64p, _f64p]
+    lib.sp_tri_get_state.argtypes = [c_ctx, _f64p, _f64p, _f64p]
+    lib.sp_tri_get_flow.argtypes = [c_ctx, _f64p, _f64p]
+    lib.sp_tri_compute_step.argtypes = [c_ctx]
+    lib.sp_tri_finalize_step.argtypes = [c_ctx, ctypes.c_double]
+    lib.sp_tri_is_diverged.argtypes = [c_ctx, _i32p]
+    lib.sp_tri_measure_correlation.argtypes = [c_ctx, _f64p]
+    lib.sp_tri_synchronize.argtypes = [c_ctx]
+    lib.sp_tri_last_error.argtypes = [c_ctx]
+    lib.sp_tri_last_error.restype = ctypes.c_char_p
+    lib.sp_tri_profile_enable.argtypes = [c_ctx, ctypes.c_int]
+    lib.sp_tri_profile_get.argtypes = [c_ctx, ctypes.c_int, _f64p, ctypes.POINTER(ctypes.c_long)]
+    lib.sp_tri_ctx_footprint.argtypes = [ctypes.POINTER(SpTriProblem), ctypes.POINTER(ctypes.c_longlong)]
+
+
+class SpTriProblem(ctypes.Structure):
+    _fields_ = [("nw", ctypes.c_int), ("mesh", _f32p), ("L", ctypes.c_int),
+                ("site_rid", _i32p), ("site_perm", _i32p), ("inv_rid", _i32p), ("inv_perm", _i32p),
+                ("ovl_off", _i32p), ("ovl_rid1", _i32p), ("ovl_rid2", _i32p), ("ovl_perm1", _i32p),
+                ("ovl_perm2", _i32p), ("R", ctypes.c_int), ("range_fwd", _i32p), ("range_inv", _i32p),
+                ("B", ctypes.c_int), ("Rb", ctypes.c_int), ("corr_rid", _i32p), ("corr_perm", _i32p)]
+
+
+class TriProblemBuffers:
+    """Keeps the numpy arrays alive behind an ``sp_tri_problem`` struct."""
+
+    def __init__(self, prob):
+        keep = [np.ascontiguousarray(prob.mesh, dtype=np.float32)]
+        for a in (prob.site_rid, prob.site_perm, prob.inv_rid, prob.inv_perm, prob.ovl_off, prob.ovl_rid1,
+                  prob.ovl_rid2, prob.ovl_perm1, prob.ovl_perm2, prob.range_fwd, prob.range_inv, prob.corr_rid,
+                  prob.corr_perm):
+            keep.append(np.ascontiguousarray(a, dtype=np.int32))
+        self.keep = keep
+        k = keep
+        self.struct = SpTriProblem(prob.nw, _p(k[0], _f32p), prob.L, _p(k[1], _i32p), _p(k[2], _i32p),
+                                   _p(k[3], _i32p), _p(k[4], _i32p), _p(k[5], _i32p), _p(k[6], _i32p),
+                                   _p(k[7], _i32p), _p(k[8], _i32p), _p(k[9], _i32p), len(k[10]), _p(k[10], _i32p),
+                                   _p(k[11], _i32p), prob.B, prob.Rb, _p(k[12], _i32p), _p(k[13], _i32p))
+
+
+def tri_footprint_bytes(prob) -> int:
+    lib = load_library()
+    buf = TriProblemBuffers(prob)
+    out = ctypes.c_longlong(0)
+    rc = lib.sp_tri_ctx_footprint(ctypes.byref(buf.struct), ctypes.byref(out))
+    if rc != SP_OK:
+        raise FlowError(rc, "sp_tri_ctx_footprint failed")
+    return int(out.value)
+
+
+class TriFlowContext:
+    """One resident TRI flow on the GPU (owner of an ``sp_tri_ctx``)."""
+
+    def __init__(self, prob, lib: Optional[ctypes.CDLL] = None):
+        self.lib = lib or load_library()
+        self.prob = prob
+        self._buf = TriProblemBuffers(prob)
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.sp_tri_ctx_create(ctypes.byref(self._buf.struct), ctypes.byref(self.ctx))
+        if rc != SP_OK:
+            msg = self.lib.sp_tri_last_error(self.ctx).decode() if self.ctx else "sp_tri_ctx_create failed"
+            if self.ctx:
+                self.lib.sp_tri_ctx_destroy(self.ctx)
+                self.ctx = ctypes.c_void_p()
+            raise FlowError(rc, msg)
+
+    def _check(self, rc: int, what: str):
+        if rc != SP_OK:
+            raise FlowError(rc, "%s: %s" % (what, self.lib.sp_tri_last_error(self.ctx).decode()))
+
+    def close(self):
+        if self.ctx:
+            self.lib.sp_tri_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, cutoff: float, v2, v4):
+        v2, v4 = (np.ascontiguousarray(x, dtype=np.float64) for x in (v2, v4))
+        if v2.shape != (self.prob.nw,) or v4.shape != (self.prob.vertex_size,):
+            raise FlowError(SP_EARG, "state arrays have the wrong size")
+        self._check(self.lib.sp_tri_set_state(self.ctx, float(cutoff), _p(v2, _f64p), _p(v4, _f64p)), "sp_tri_set_state")
+
+    def get_state(self):
+        c = ctypes.c_double(0)
+        v2 = np.zeros(self.prob.nw)
+        v4 = np.zeros(self.prob.vertex_size)
+        self._check(self.lib.sp_tri_get_state(self.ctx, ctypes.byref(c), _p(v2, _f64p), _p(v4, _f64p)),
+                    "sp_tri_get_state")
+        return c.value, v2, v4
+
+    def get_flow(self):
+        dv2 = np.zeros(self.prob.nw)
+        dv4 = np.zeros(self.prob.vertex_size)
+        self._check(self.lib.sp_tri_get_flow(self.ctx, _p(dv2, _f64p), _p(dv4, _f64p)), "sp_tri_get_flow")
+        return dv2, dv4
+
+    def compute_step(self):
+        self._check(self.lib.sp_tri_compute_step(self.ctx), "sp_tri_compute_step")
+
+    def finalize_step(self, new_cutoff: float):
+        self._check(self.lib.sp_tri_finalize_step(self.ctx, float(new_cutoff)), "sp_tri_finalize_step")
+
+    def is_diverged(self) -> bool:
+        d = ctypes.c_int(0)
+        self._check(self.lib.sp_tri_is_diverged(self.ctx, ctypes.byref(d)), "sp_tri_is_diverged")
+        return bool(d.value)
+
+    def measure_correlation(self):
+        """(10, B, Rb) array in the group order of TRI_GROUPS."""
+        n = 10 * self.prob.B * self.prob.Rb
+        out = np.zeros(n)
+        self._check(self.lib.sp_tri_measure_correlation(self.ctx, _p(out, _f64p)), "sp_tri_measure_correlation")
+        return out.reshape(10, self.prob.B, self.prob.Rb)
+
+    def synchronize(self):
+        self._check(self.lib.sp_tri_synchronize(self.ctx), "sp_tri_synchronize")
+
+    def profile(self, enable: bool):
+        self._check(self.lib.sp_tri_profile_enable(self.ctx, 1 if enable else 0), "sp_tri_profile_enable")
+
+    def profile_get(self, kernel: int) -> Tuple[float, int]:
+        ms = ctypes.c_double(0)
+        n = ctypes.c_long(0)
+        self._check(self.lib.sp_tri_profile_get(self.ctx, kernel, ctypes.byref(ms), ctypes.byref(n)),
+                    "sp_tri_profile_get")
+        return ms.value, n.value

@@ -1,0 +1,184 @@
+// Device helpers shared by the SU(2) and TRI flow kernels: the frequency mesh
+// (bracket searches, interpolation), the self-energy interpolation, the
+// trapezoid evaluation points of ImplicitIntegrator and the per-cutoff tables
+// of evaluation points and propagator weights.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace frg {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2.0 * kPi;
+
+// ---------------------------------------------------------------------------
+// frequency mesh (positive half m[0..n) ascending; signed index i in [-n, n))
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mesh_at(const double *m, int i) { return i >= 0 ? m[i] : -m[-i - 1]; }
+
+__device__ inline int greater_pos(const double *m, int n, double w) {
+  if (w <= m[0]) return 0;
+  for (int i = 1; i < n; ++i)
+    if (m[i] > w) return i;
+  return n - 1;
+}
+
+__device__ inline int lesser_pos(const double *m, int n, double w) {
+  if (w <= m[0]) return 0;
+  for (int i = 1; i < n; ++i)
+    if (m[i] > w) return i - 1;
+  return n - 1;
+}
+
+// FrequencyDiscretization::lesser / greater (src/FrequencyDiscretization.hpp:253-296)
+__device__ inline int mesh_lesser(const double *m, int n, double w) { return w < 0 ? -(greater_pos(m, n, -w) + 1) : lesser_pos(m, n, w); }
+__device__ inline int mesh_greater(const double *m, int n, double w) { return w < 0 ? -(lesser_pos(m, n, -w) + 1) : greater_pos(m, n, w); }
+
+// number of mesh points <= w (branch-free binary lifting)
+__device__ __forceinline__ int count_le(const double *m, int n, double w) {
+  int pos = 0;
+  for (int step = 1 << (31 - __clz(n)); step > 0; step >>= 1) {
+    const int p = pos + step;
+    if (p <= n && m[p - 1] <= w) pos = p;
+  }
+  return pos;
+}
+
+// FrequencyDiscretization::interpolateOffset (src/FrequencyDiscretization.hpp:326-356)
+// (written as !(w > m[0]) so that a NaN argument clamps to index 0 instead of
+// producing index -1; identical to the reference for every non-NaN w)
+__device__ __forceinline__ void interp(const double *m, int n, double w, int &lo, int &hi, double &bias) {
+  if (!(w > m[0])) { lo = 0; hi = 0; bias = 0.0; return; }
+  const int c = count_le(m, n, w);
+  if (c >= n) { lo = n - 1; hi = n - 1; bias = 0.0; return; }
+  lo = c - 1;
+  hi = c;
+  bias = (w - m[lo]) / (m[hi] - m[lo]);
+}
+
+// SU2VertexSingleParticle::getValue (src/SU2/SU2VertexSingleParticle.hpp:73-87)
+__device__ inline double sigma_at(const double *m, int n, const double *data, double w) {
+  double sign = 1.0;
+  if (w < 0) { w = -w; sign = -1.0; }
+  int lo, hi; double b;
+  interp(m, n, w, lo, hi, b);
+  return sign * ((1 - b) * data[lo] + b * data[hi]);
+}
+
+__device__ __forceinline__ int tri(int a, int b) { return a * (a + 1) / 2 + b; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// trapezoid windows: evaluation points and weights (0.5 folded in), in the
+// order ImplicitIntegrator visits them (src/lib/Integrator.hpp:137-287)
+// ---------------------------------------------------------------------------
+struct PointList {
+  double *wp;
+  double *w;
+  int cnt;
+  int cap;
+  __device__ inline void push(double x, double weight) {
+    if (cnt < cap) { wp[cnt] = x; w[cnt] = weight; }
+    ++cnt;
+  }
+};
+
+__device__ inline void window_right(const double *m, int n, double max, PointList &o) {
+  int umin = -n;
+  const int umax = mesh_lesser(m, n, max);
+  if (umin != umax) {
+    o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - mesh_at(m, umin)));
+    while (++umin != umax) o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - mesh_at(m, umin - 1)));
+    o.push(mesh_at(m, umin), 0.5 * (max - mesh_at(m, umin - 1)));
+    o.push(max, 0.5 * (max - mesh_at(m, umin)));
+  } else {
+    const double h = 0.5 * (max - mesh_at(m, umin));
+    o.push(max, h);
+    o.push(mesh_at(m, umin), h);
+  }
+}
+
+__device__ inline void window_both(const double *m, int n, double min, double max, PointList &o) {
+  int umin = mesh_greater(m, n, min);
+  const int umax = mesh_lesser(m, n, max);
+  if (umax >= umin) {
+    o.push(min, 0.5 * (mesh_at(m, umin) - min));
+    if (umax != umin) {
+      o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - min));
+      while (++umin != umax) o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - mesh_at(m, umin - 1)));
+      o.push(mesh_at(m, umin), 0.5 * (max - mesh_at(m, umin - 1)));
+    } else {
+      o.push(mesh_at(m, umin), 0.5 * (max - min));
+    }
+    o.push(max, 0.5 * (max - mesh_at(m, umin)));
+  } else {
+    const double h = 0.5 * (max - min);
+    o.push(max, h);
+    o.push(min, h);
+  }
+}
+
+__device__ inline void window_left(const double *m, int n, double min, PointList &o) {
+  int umin = mesh_greater(m, n, min);
+  const int max = n - 1;
+  if (umin != max) {
+    o.push(min, 0.5 * (mesh_at(m, umin) - min));
+    o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - min));
+    while (++umin != max) o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - mesh_at(m, umin - 1)));
+    o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin) - mesh_at(m, umin - 1)));
+  } else {
+    const double h = 0.5 * (mesh_at(m, umin) - min);
+    o.push(min, h);
+    o.push(mesh_at(m, umin), h);
+  }
+}
+
+// the three Katanin windows of a channel with transfer frequency X
+// (src/SU2/SU2FrgCore.cpp:378-392; with X = nu = 0 the correlation windows,
+// src/SU2/SU2MeasurementCorrelation.cpp:145-159)
+__device__ inline void katanin_windows(const double *m, int n, double X, double cut, PointList &o) {
+  if (-(X + cut) > mesh_at(m, -n)) window_right(m, n, -(X + cut), o);
+  if (X - cut > cut) window_both(m, n, cut - X, -cut, o);
+  if (cut < mesh_at(m, n - 1)) window_left(m, n, cut, o);
+}
+
+// Evaluation points and weights of one channel with exact transfer frequency
+// m[x] at cutoff cut: the conventional points +-cut with p(cut, cut +- X)
+// (src/SU2/SU2FrgCore.cpp:349-371, src/TRI/TRIFrgCore.cpp:2917-2940), then the
+// Katanin windows with dSigma(w') G(w')^2 G(X + w') folded into the trapezoid
+// weight (src/SU2/SU2FrgCore.cpp:343-347,373-424).  Depends on (x, cut) only,
+// so every item of a flow shares the table.  *overflow is raised if the list
+// exceeds emax entries.
+__device__ inline void channel_points(int x, const double *m, int nw, double cut, const double *v2, const double *dv2,
+                                      double *Wwp, double *Wc, int *Wn, int emax, int *overflow) {
+  const double X = m[x];
+  PointList o{Wwp + (size_t)x * emax, Wc + (size_t)x * emax, 0, emax};
+  auto p = [&](double a, double b) { return 1.0 / ((a + sigma_at(m, nw, v2, a)) * (b + sigma_at(m, nw, v2, b))); };
+  o.push(cut, p(cut, cut + X));
+  if (X > 2.0 * cut) o.push(-cut, p(cut, cut - X));
+  const int k0 = o.cnt;
+  katanin_windows(m, nw, X, cut, o);
+  const int kend = o.cnt < emax ? o.cnt : emax;
+  for (int k = k0; k < kend; ++k) {
+    const double wp = o.wp[k];
+    const double d1 = wp + sigma_at(m, nw, v2, wp);
+    o.w[k] *= sigma_at(m, nw, dv2, wp) / (d1 * d1 * (X + wp + sigma_at(m, nw, v2, X + wp)));
+  }
+  Wn[x] = kend;
+  if (o.cnt > emax) atomicOr(overflow, 1);
+}
+
+// Plain trapezoid points of the correlation integrals at nu = 0
+// (src/SU2/SU2MeasurementCorrelation.cpp:145-159, src/TRI/TRIMeasurementCorrelation.cpp:270-284)
+__device__ inline void correlation_points(const double *m, int nw, double cut, double *Cwp, double *Cw, int *Cn,
+                                          int emax, int *overflow) {
+  PointList o{Cwp, Cw, 0, emax};
+  katanin_windows(m, nw, 0.0, cut, o);
+  *Cn = o.cnt < emax ? o.cnt : emax;
+  if (o.cnt > emax) atomicOr(overflow, 1);
+}
+
+}  // namespace frg

@@ -58,6 +58,7 @@
 #include <vector>
 
 #include "spfrg.h"
+#include "frg_common.h"
 
 namespace {
 
@@ -75,8 +76,6 @@ namespace {
 #define SPFRG_MIN_BLOCKS_MIXED 1
 #endif
 constexpr int kThreads = SPFRG_THREADS;
-constexpr double kPi = 3.14159265358979323846;
-constexpr double kTwoPi = 2.0 * kPi;
 constexpr int kMaxBatch = SPFRG_MAX_BATCH;
 #ifndef SPFRG_UNROLL_FP64
 #define SPFRG_UNROLL_FP64 1
@@ -89,61 +88,8 @@ constexpr int kMaxBatch = SPFRG_MAX_BATCH;
 #endif
 constexpr long long kChunk = SPFRG_CHUNK;
 
-// ---------------------------------------------------------------------------
-// frequency mesh (positive half m[0..n) ascending; signed index i in [-n, n))
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double mesh_at(const double *m, int i) { return i >= 0 ? m[i] : -m[-i - 1]; }
+using namespace frg;
 
-__device__ int greater_pos(const double *m, int n, double w) {
-  if (w <= m[0]) return 0;
-  for (int i = 1; i < n; ++i)
-    if (m[i] > w) return i;
-  return n - 1;
-}
-
-__device__ int lesser_pos(const double *m, int n, double w) {
-  if (w <= m[0]) return 0;
-  for (int i = 1; i < n; ++i)
-    if (m[i] > w) return i - 1;
-  return n - 1;
-}
-
-// FrequencyDiscretization::lesser / greater (src/FrequencyDiscretization.hpp:253-296)
-__device__ int mesh_lesser(const double *m, int n, double w) { return w < 0 ? -(greater_pos(m, n, -w) + 1) : lesser_pos(m, n, w); }
-__device__ int mesh_greater(const double *m, int n, double w) { return w < 0 ? -(lesser_pos(m, n, -w) + 1) : greater_pos(m, n, w); }
-
-// number of mesh points <= w (branch-free binary lifting)
-__device__ __forceinline__ int count_le(const double *m, int n, double w) {
-  int pos = 0;
-  for (int step = 1 << (31 - __clz(n)); step > 0; step >>= 1) {
-    const int p = pos + step;
-    if (p <= n && m[p - 1] <= w) pos = p;
-  }
-  return pos;
-}
-
-// FrequencyDiscretization::interpolateOffset (src/FrequencyDiscretization.hpp:326-356)
-// (written as !(w > m[0]) so that a NaN argument clamps to index 0 instead of
-// producing index -1; identical to the reference for every non-NaN w)
-__device__ __forceinline__ void interp(const double *m, int n, double w, int &lo, int &hi, double &bias) {
-  if (!(w > m[0])) { lo = 0; hi = 0; bias = 0.0; return; }
-  const int c = count_le(m, n, w);
-  if (c >= n) { lo = n - 1; hi = n - 1; bias = 0.0; return; }
-  lo = c - 1;
-  hi = c;
-  bias = (w - m[lo]) / (m[hi] - m[lo]);
-}
-
-// SU2VertexSingleParticle::getValue (src/SU2/SU2VertexSingleParticle.hpp:73-87)
-__device__ double sigma_at(const double *m, int n, const double *data, double w) {
-  double sign = 1.0;
-  if (w < 0) { w = -w; sign = -1.0; }
-  int lo, hi; double b;
-  interp(m, n, w, lo, hi, b);
-  return sign * ((1 - b) * data[lo] + b * data[hi]);
-}
-
-__device__ __forceinline__ int tri(int a, int b) { return a * (a + 1) / 2 + b; }
 
 // row index of a support point with the s <-> u exchange of
 // SU2VertexTwoParticle::_generateAccessBufferOffset (:616-632); neg = density sign flip
@@ -192,85 +138,6 @@ __device__ double2 value_none(const double2 *G, const double *m, int nw, int L, 
   return r;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// ---------------------------------------------------------------------------
-// trapezoid windows: evaluation points and weights (0.5 folded in), in the
-// order ImplicitIntegrator visits them (src/lib/Integrator.hpp:137-287)
-// ---------------------------------------------------------------------------
-struct PointList {
-  double *wp;
-  double *w;
-  int cnt;
-  int cap;
-  __device__ void push(double x, double weight) {
-    if (cnt < cap) { wp[cnt] = x; w[cnt] = weight; }
-    ++cnt;
-  }
-};
-
-__device__ void window_right(const double *m, int n, double max, PointList &o) {
-  int umin = -n;
-  const int umax = mesh_lesser(m, n, max);
-  if (umin != umax) {
-    o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - mesh_at(m, umin)));
-    while (++umin != umax) o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - mesh_at(m, umin - 1)));
-    o.push(mesh_at(m, umin), 0.5 * (max - mesh_at(m, umin - 1)));
-    o.push(max, 0.5 * (max - mesh_at(m, umin)));
-  } else {
-    const double h = 0.5 * (max - mesh_at(m, umin));
-    o.push(max, h);
-    o.push(mesh_at(m, umin), h);
-  }
-}
-
-__device__ void window_both(const double *m, int n, double min, double max, PointList &o) {
-  int umin = mesh_greater(m, n, min);
-  const int umax = mesh_lesser(m, n, max);
-  if (umax >= umin) {
-    o.push(min, 0.5 * (mesh_at(m, umin) - min));
-    if (umax != umin) {
-      o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - min));
-      while (++umin != umax) o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - mesh_at(m, umin - 1)));
-      o.push(mesh_at(m, umin), 0.5 * (max - mesh_at(m, umin - 1)));
-    } else {
-      o.push(mesh_at(m, umin), 0.5 * (max - min));
-    }
-    o.push(max, 0.5 * (max - mesh_at(m, umin)));
-  } else {
-    const double h = 0.5 * (max - min);
-    o.push(max, h);
-    o.push(min, h);
-  }
-}
-
-__device__ void window_left(const double *m, int n, double min, PointList &o) {
-  int umin = mesh_greater(m, n, min);
-  const int max = n - 1;
-  if (umin != max) {
-    o.push(min, 0.5 * (mesh_at(m, umin) - min));
-    o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - min));
-    while (++umin != max) o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin + 1) - mesh_at(m, umin - 1)));
-    o.push(mesh_at(m, umin), 0.5 * (mesh_at(m, umin) - mesh_at(m, umin - 1)));
-  } else {
-    const double h = 0.5 * (mesh_at(m, umin) - min);
-    o.push(min, h);
-    o.push(mesh_at(m, umin), h);
-  }
-}
-
-// the three Katanin windows of a channel with transfer frequency X
-// (src/SU2/SU2FrgCore.cpp:378-392; with X = nu = 0 the correlation windows,
-// src/SU2/SU2MeasurementCorrelation.cpp:145-159)
-__device__ void katanin_windows(const double *m, int n, double X, double cut, PointList &o) {
-  if (-(X + cut) > mesh_at(m, -n)) window_right(m, n, -(X + cut), o);
-  if (X - cut > cut) window_both(m, n, cut - X, -cut, o);
-  if (cut < mesh_at(m, n - 1)) window_left(m, n, cut, o);
-}
-
 // ---------------------------------------------------------------------------
 // per-flow descriptor
 // ---------------------------------------------------------------------------
@@ -309,36 +176,13 @@ __device__ __forceinline__ int flow_of(const FlowDev *__restrict__ fl, int nflow
 __global__ void k_weights(const FlowDev *__restrict__ fl, int corr_only) {
   const FlowDev &F = fl[blockIdx.y];
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nw = F.nw, emax = F.emax;
-  const double *m = F.mesh;
-  const double cut = F.cut;
-  if (x > nw) return;
-  if (x == nw) {
-    PointList o{F.Cwp, F.Cw, 0, emax};
-    katanin_windows(m, nw, 0.0, cut, o);
-    *F.Cn = o.cnt < emax ? o.cnt : emax;
-    if (o.cnt > emax) atomicOr(&F.flags[1], 1);
+  if (x > F.nw) return;
+  if (x == F.nw) {
+    correlation_points(F.mesh, F.nw, F.cut, F.Cwp, F.Cw, F.Cn, F.emax, F.flags + 1);
     return;
   }
   if (corr_only) return;
-  const double *v2 = F.v2, *dv2 = F.dv2;
-  const double X = m[x];
-  PointList o{F.Wwp + (size_t)x * emax, F.Wc + (size_t)x * emax, 0, emax};
-  // conventional contribution (src/SU2/SU2FrgCore.cpp:349-371): p(L, L +- X)
-  auto p = [&](double a, double b) { return 1.0 / ((a + sigma_at(m, nw, v2, a)) * (b + sigma_at(m, nw, v2, b))); };
-  o.push(cut, p(cut, cut + X));
-  if (X > 2.0 * cut) o.push(-cut, p(cut, cut - X));
-  const int k0 = o.cnt;
-  katanin_windows(m, nw, X, cut, o);
-  const int kend = o.cnt < emax ? o.cnt : emax;
-  // Katanin propagator dSigma(w') G(w')^2 G(X + w') (src/SU2/SU2FrgCore.cpp:343-347)
-  for (int k = k0; k < kend; ++k) {
-    const double wp = o.wp[k];
-    const double d1 = wp + sigma_at(m, nw, v2, wp);
-    o.w[k] *= sigma_at(m, nw, dv2, wp) / (d1 * d1 * (X + wp + sigma_at(m, nw, v2, X + wp)));
-  }
-  F.Wn[x] = kend;
-  if (o.cnt > emax) atomicOr(&F.flags[1], 1);
+  channel_points(x, F.mesh, F.nw, F.cut, F.v2, F.dv2, F.Wwp, F.Wc, F.Wn, F.emax, F.flags + 1);
 }
 
 // ---------------------------------------------------------------------------

@@ -9,8 +9,8 @@ same way ``SpinParser::runCore`` drives it:
   ``finalizeStep(newCutoff)``, ``takeMeasurements()``, ``flowingFunctional()``,
   ``flow()``;
 * ``FrgCoreFactory.newFrgCore(identifier, ...)`` (``src/FrgCoreFactory.cpp:25-52``)
-  selects the GPU core for ``symmetry="SU2"`` when the model carries the option
-  ``<backend>hip</backend>`` (or when ``backend="hip"`` is passed);
+  selects the GPU core for ``symmetry="SU2"`` or ``"TRI"`` when the model carries
+  the option ``<backend>hip</backend>`` (or when ``backend="hip"`` is passed);
 * ``run_core`` restates the cutoff loop of ``src/SpinParser.cpp:126-186``.
 """
 from __future__ import annotations
@@ -20,8 +20,8 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from .capi import FlowContext, SP_PRECISION_FP64
-from .problem import SU2Problem, su2_problem
+from .capi import TRI_GROUPS, FlowContext, SP_PRECISION_FP64, TriFlowContext
+from .problem import SU2Problem, TRIProblem, su2_problem, tri_problem
 from .task import MeasurementSpec, Task
 
 
@@ -54,10 +54,10 @@ class EffectiveAction:
         return self._ctx.is_diverged()
 
     def arrays(self):
-        """Download (v2, v4ss, v4dd) in the reference layout."""
+        """Download the vertex arrays in the reference layout: (v2, v4ss, v4dd)
+        for the SU(2) core, (v2, v4) for the TRI core."""
         if self._which == "state":
-            _, v2, v4s, v4d = self._ctx.get_state()
-            return v2, v4s, v4d
+            return tuple(self._ctx.get_state()[1:])
         return self._ctx.get_flow()
 
 
@@ -101,6 +101,31 @@ class SU2MeasurementCorrelation(Measurement):
         zz, dd = core.context.measure_correlation()
         if isMasterTask:
             self.records.append(CorrelationRecord(cutoff, zz, dd))
+
+
+@dataclass
+class TRICorrelationRecord:
+    cutoff: float
+    data: Dict[str, np.ndarray]   # TRICorDD, TRICorXX, ... TRICorZZ: (basis, sites in range)
+
+
+class TRIMeasurementCorrelation(Measurement):
+    """GPU correlation measurement of the TRI core
+    (``src/TRI/TRIMeasurementCorrelation.cpp:147-316``): the 10 groups
+    ``TRICorDD``, ``TRICorXX`` ... ``TRICorZZ``; duplicate cutoffs are discarded
+    like the reference does."""
+
+    def __init__(self, spec: MeasurementSpec):
+        super().__init__(spec)
+        self.records: List[TRICorrelationRecord] = []
+
+    def takeMeasurement(self, core: "HipTRIFrgCore", isMasterTask: bool = True) -> None:
+        cutoff = core.cutoff
+        if self.records and np.float32(self.records[-1].cutoff) == np.float32(cutoff):
+            return
+        out = core.context.measure_correlation()
+        if isMasterTask:
+            self.records.append(TRICorrelationRecord(cutoff, {g: out[i] for i, g in enumerate(TRI_GROUPS)}))
 
 
 class FrgCore:
@@ -166,9 +191,49 @@ class HipSU2FrgCore(FrgCore):
         self.context.close()
 
 
+class HipTRIFrgCore(FrgCore):
+    """TRI core on the GPU (``src/TRI/TRIFrgCore.cpp``): 16-component vertex
+    resident in HBM, FP64."""
+
+    def __init__(self, problem: TRIProblem, measurements: List[Measurement]):
+        super().__init__(measurements)
+        self.problem = problem
+        self.normalization = float(problem.normalization)
+        self.context = TriFlowContext(problem)
+        self.cutoff = float(problem.cutoffs[0])
+        v2, v4 = problem.initial_state(np.float64)
+        self.context.set_state(self.cutoff, v2, v4)
+        self._state = EffectiveAction(self.context, "state")
+        self._state._core_cutoff = self.cutoff
+        self._flow = EffectiveAction(self.context, "flow")
+
+    def computeStep(self) -> None:
+        self.context.compute_step()
+        self._flow._cutoff = self.cutoff
+
+    def finalizeStep(self, newCutoff: float) -> None:
+        self.context.finalize_step(float(newCutoff))
+        self.cutoff = float(newCutoff)
+        self._state._core_cutoff = self.cutoff
+
+    def flowingFunctional(self) -> EffectiveAction:
+        return self._state
+
+    def flow(self) -> EffectiveAction:
+        return self._flow
+
+    def setState(self, cutoff: float, v2, v4) -> None:
+        self.context.set_state(cutoff, v2, v4)
+        self.cutoff = float(cutoff)
+        self._state._core_cutoff = self.cutoff
+
+    def close(self) -> None:
+        self.context.close()
+
+
 class FrgCoreFactory:
     @staticmethod
-    def newFrgCore(identifier: str, problem: SU2Problem, measurements: List[MeasurementSpec],
+    def newFrgCore(identifier: str, problem, measurements: List[MeasurementSpec],
                    options: Optional[Dict[str, str]] = None, backend: Optional[str] = None) -> FrgCore:
         options = dict(options or {})
         backend = backend or options.pop("backend", None)
@@ -176,18 +241,26 @@ class FrgCoreFactory:
         for spec in measurements:
             if spec.identifier != "correlation":
                 raise InitializationError("Measurement: Unknown measurement type '%s'." % spec.identifier)
-            if identifier != "SU2":
+            if identifier == "SU2":
+                objs.append(SU2MeasurementCorrelation(spec))
+            elif identifier == "TRI":
+                objs.append(TRIMeasurementCorrelation(spec))
+            else:
                 raise InitializationError("Measurement [correlation]: no GPU measurement for '%s'." % identifier)
-            objs.append(SU2MeasurementCorrelation(spec))
         if identifier == "SU2" and backend == "hip":
             return HipSU2FrgCore(problem, objs)
+        if identifier == "TRI" and backend == "hip":
+            return HipTRIFrgCore(problem, objs)
         if identifier in ("SU2", "XYZ", "TRI"):
             raise ArgumentError("Core '%s' with backend '%s' is not provided by this engine" % (identifier, backend))
         raise ArgumentError("Spin model identifier '%s' does not exist." % identifier)
 
 
-def new_core_from_task(task: Task, resource_path: Optional[str] = None, backend: str = "hip") -> HipSU2FrgCore:
-    prob = su2_problem(task, resource_path)
+def new_core_from_task(task: Task, resource_path: Optional[str] = None, backend: str = "hip") -> FrgCore:
+    if task.symmetry == "TRI":
+        prob = tri_problem(task, resource_path)
+    else:
+        prob = su2_problem(task, resource_path)
     options = {k: v for k, v in task.model_options.items() if k in ("spin", "normalization", "backend")}
     return FrgCoreFactory.newFrgCore(task.symmetry, prob, task.measurements or [MeasurementSpec("correlation")],
                                      options, backend=backend)

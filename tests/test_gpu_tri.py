@@ -1,0 +1,127 @@
+"""GPU parity of the TRI core (HIP path through the C ABI) against the FP64 CPU oracle.
+
+Bar: 1e-10 normwise relative error (max|gpu - oracle| / max|oracle|) on dSigma,
+dGamma and all 10 correlation outputs; the full golden flow additionally within
+the reference's own 1e-5 absolute tolerance of its float golden file
+(test/scripted/assets/test_eval.py:14,28-29).
+
+Cases: the reference's TRI golden problem (honeycomb Heisenberg-Kitaev-Gamma,
+range 3, 14 frequencies) at oracle-advanced states, and the kagome lattice with
+out-of-plane Dzyaloshinskii-Moriya coupling D = 0.2 at range 5 (L = 19, 771
+overlap terms, spin permutations on half of the site transformations) on a
+reduced frequency mesh.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, cfg4_problem, max_rel, tri_ref3_problem
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ref3():
+    from oracle import TriOracle
+    p = tri_ref3_problem()
+    return p, TriOracle(p)
+
+
+def _advanced_state(o, p, k):
+    """FP64 oracle state after k Euler steps from the initial condition."""
+    if k == 0:
+        v2, v4 = p.initial_state(np.float64)
+        return float(p.cutoffs[0]), v2, v4
+    _, n, (v2, v4) = o.run_flow(dtype=np.float64, nthreads=8, ncut=k + 1)
+    assert n == k + 1
+    return float(p.cutoffs[k]), v2, v4
+
+
+@pytest.mark.parametrize("k", [0, 12, 25, 33])
+def test_tri_step_parity_golden_problem(ref3, k):
+    from spinparser_amd.capi import TriFlowContext
+    p, o = ref3
+    cut, v2, v4 = _advanced_state(o, p, k)
+    _log("TRI step parity: honeycomb-kitaev-gamma r3 nw=%d L=%d at cutoff index %d (%.4g)" % (p.nw, p.L, k, cut))
+    dv2_o, dv4_o, _ = o.compute_step(cut, v2, v4, dtype=np.float64, nthreads=8)
+    ctx = TriFlowContext(p)
+    ctx.set_state(cut, v2, v4)
+    ctx.compute_step()
+    dv2_g, dv4_g = ctx.get_flow()
+    assert not ctx.is_diverged()
+    e2, e4 = max_rel(dv2_g, dv2_o), max_rel(dv4_g, dv4_o)
+    _log("  dSigma %.3g  dGamma %.3g" % (e2, e4))
+    assert e2 < TOL and e4 < TOL
+    corr_g = ctx.measure_correlation()
+    corr_o = o.correlation(cut, v2, v4, dtype=np.float64)
+    from oracle import TRI_GROUPS
+    ec = max(max_rel(corr_g[i], corr_o[g]) for i, g in enumerate(TRI_GROUPS))
+    _log("  correlations %.3g" % ec)
+    assert ec < TOL
+    ctx.close()
+
+
+def test_tri_full_flow_golden(ref3):
+    """Whole flow on the GPU: every record within 1e-10 of the FP64 oracle flow
+    and within the reference's 1e-5 of its float golden file."""
+    from spinparser_amd.capi import TriFlowContext
+    from oracle import TRI_GROUPS
+    p, o = ref3
+    rec_o, n_o, _ = o.run_flow(dtype=np.float64, nthreads=8)
+    ctx = TriFlowContext(p)
+    v2, v4 = p.initial_state(np.float64)
+    ctx.set_state(float(p.cutoffs[0]), v2, v4)
+    recs = []
+    for k in range(len(p.cutoffs) - 1):
+        ctx.compute_step()
+        recs.append(ctx.measure_correlation())
+        assert not ctx.is_diverged()
+        ctx.finalize_step(float(p.cutoffs[k + 1]))
+    recs.append(ctx.measure_correlation())
+    ctx.close()
+    assert len(recs) == n_o
+    g = np.load(GOLDEN + "/reference3_tri.npz")
+    worst_o = worst_g = 0.0
+    for k in range(n_o):
+        worst_o = max(worst_o, max_rel(recs[k], rec_o[k]))
+        for i, grp in enumerate(TRI_GROUPS):
+            worst_g = max(worst_g, float(np.abs(recs[k][i] - g["%s/data/measurement_%d/data" % (grp, k)]).max()))
+    _log("TRI full flow: %d records, vs FP64 oracle %.3g, vs float golden %.3g (abs)" % (n_o, worst_o, worst_g))
+    assert worst_o < TOL
+    assert worst_g < 1e-5
+
+
+def test_tri_kagome_dm_parity():
+    """Kagome DM range 5 (spin-permuted site transformations) on an 8-point mesh."""
+    from spinparser_amd.capi import TriFlowContext
+    from oracle import TRI_GROUPS, TriOracle
+    p = cfg4_problem(nw=8, lattice_range=5)
+    o = TriOracle(p)
+    for k in (0, 150, 300):
+        cut, v2, v4 = _advanced_state(o, p, 0)
+        cut = float(p.cutoffs[k])
+        if k > 0:
+            # a frequency-dependent state: one oracle Euler step of the initial condition
+            dv2, dv4, _ = o.compute_step(cut, v2, v4, dtype=np.float64, nthreads=8)
+            v2, v4 = v2 - 0.05 * cut * dv2, v4 - 0.05 * cut * dv4
+        _log("TRI kagome-DM r5 nw=%d L=%d O=%d at cutoff %.4g" % (p.nw, p.L, p.overlap_total, cut))
+        dv2_o, dv4_o, _ = o.compute_step(cut, v2, v4, dtype=np.float64, nthreads=8)
+        ctx = TriFlowContext(p)
+        ctx.set_state(cut, v2, v4)
+        ctx.compute_step()
+        dv2_g, dv4_g = ctx.get_flow()
+        e2, e4 = max_rel(dv2_g, dv2_o), max_rel(dv4_g, dv4_o)
+        corr_g = ctx.measure_correlation()
+        corr_o = o.correlation(cut, v2, v4, dtype=np.float64)
+        ec = max(max_rel(corr_g[i], corr_o[g]) for i, g in enumerate(TRI_GROUPS))
+        _log("  dSigma %.3g  dGamma %.3g  correlations %.3g" % (e2, e4, ec))
+        ctx.close()
+        assert e2 < TOL and e4 < TOL and ec < TOL
