@@ -31,6 +31,13 @@ dGamma/dLambda evaluations per second (``value``, FP64 form).  Beside it:
   on a bounded sample of the same workload, extrapolated by integrand-call
   counts over the schedule.
 
+* ``tri``: the TRI core on kagome J = 1, D = 0.2 (range 5, 48 frequencies,
+  cutoff 50 -> 0.01 x 0.98): evaluations at five cutoffs over the schedule and
+  the full-schedule rate integrated by algorithmic FLOP, with its own CPU
+  baseline (float TRI oracle on a bounded item sample).
+
+Mode "tri-flow": the whole TRI flow (421 evaluations) measured end to end.
+
 Mode "sweep": the served J2/J1 phase-diagram sweep (64 SU(2) cubic flows, J2 ~
 U[0,1], range ~ U{3..7}, N_w ~ U{32,48,64}, Poisson arrivals, mixed precision,
 admission by device footprint) against the same flows run one at a time in
@@ -49,8 +56,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from spinparser_amd.capi import (KERNEL_DGAMMA, SP_PRECISION_FP64, SP_PRECISION_MIXED,  # noqa: E402
-                                 FlowContext, footprint_bytes)
-from spinparser_amd.problem import su2_problem  # noqa: E402
+                                 FlowContext, TriFlowContext, footprint_bytes)
+from spinparser_amd.problem import su2_problem, tri_problem  # noqa: E402
 from spinparser_amd.task import make_task_xml, parse_task  # noqa: E402
 from spinparser_amd.workload import compulsory_bytes, evaluation_flops  # noqa: E402
 
@@ -66,6 +73,12 @@ WORKLOADS = {
                            freq_min=0.005, freq_max=50.0, freq_count=32, cutoff_max=50.0, cutoff_min=0.01,
                            cutoff_step=0.98),
 }
+
+# TRI core: kagome J = 1 with out-of-plane Dzyaloshinskii-Moriya D = 0.2, range
+# 5 (L = 19, 771 overlap terms), 48 log-spaced frequencies in [0.005, 50],
+# cutoff 50 -> 0.01 with step 0.98 (422 cutoffs)
+TRI_WORKLOAD = dict(lattice="kagome", lattice_range=5, model="kagome-DM", couplings={"j": 1.0, "d": 0.2},
+                    freq_min=0.005, freq_max=50.0, freq_count=48, cutoff_max=50.0, cutoff_min=0.01, cutoff_step=0.98)
 
 # HBM bytes per k_dgamma launch over the full flow, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over the 655 dispatches
@@ -83,6 +96,11 @@ TRAFFIC_KIB = {
 def build_problem(name):
     xml = make_task_xml(symmetry="SU2", **WORKLOADS[name])
     return su2_problem(parse_task(xml, is_text=True))
+
+
+def build_tri_problem():
+    xml = make_task_xml(symmetry="TRI", **TRI_WORKLOAD)
+    return tri_problem(parse_task(xml, is_text=True))
 
 
 def max_rel(a, b):
@@ -227,6 +245,130 @@ def traffic_entry(workload, dtype):
     return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
 
 
+def tri_leg(samples=5, reps=2):
+    """TRI core on the kagome-DM workload: dGamma evaluations at `samples`
+    cutoffs spread over the schedule (state = initial condition placed at the
+    cutoff; the work of an evaluation depends on the cutoff only), and the
+    full-schedule rate obtained by integrating the measured time per
+    algorithmic FLOP over the algorithmic work of every evaluation."""
+    p = build_tri_problem()
+    ncut = len(p.cutoffs)
+    ks = [int(round(x)) for x in np.linspace(0, ncut - 2, samples)]
+    v2, v4 = p.initial_state(np.float64)
+    ctx = TriFlowContext(p)
+    pts = []
+    for k in ks:
+        ctx.set_state(float(p.cutoffs[k]), v2, v4)
+        ctx.compute_step()
+        ctx.synchronize()
+        ctx.profile(True)
+        ms0, n0 = ctx.profile_get(KERNEL_DGAMMA)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.compute_step()
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        ms1, n1 = ctx.profile_get(KERNEL_DGAMMA)
+        ctx.profile(False)
+        f = evaluation_flops(p, p.cutoffs[k])
+        kern = (ms1 - ms0) * 1e-3 / max(1, n1 - n0)
+        pts.append({"cutoff": float(p.cutoffs[k]), "index": k, "gflop_per_eval": f / 1e9, "evals_per_s": 1.0 / wall,
+                    "k_tri_dgamma_ms": kern * 1e3, "tflops_k_tri_dgamma": f / kern / 1e12})
+    ctx.close()
+    # seconds per FLOP, piecewise linear in the cutoff index between samples
+    idx = np.array([q["index"] for q in pts], dtype=float)
+    spf = np.array([1.0 / (q["evals_per_s"] * q["gflop_per_eval"] * 1e9) for q in pts])
+    flops = np.array([evaluation_flops(p, p.cutoffs[k]) for k in range(ncut - 1)])
+    est = float(np.sum(flops * np.interp(np.arange(ncut - 1), idx, spf)))
+    return {"workload": "kagome-dm-r5-nw48", "L": p.L, "nw": p.nw, "overlap_terms": p.overlap_total,
+            "n_freq_triples": p.n_freq, "cutoffs": ncut, "samples": pts,
+            "full_flow_algorithmic_tflop": float(flops.sum()) / 1e12,
+            "est_full_flow_s": est, "est_evals_per_s": (ncut - 1) / est,
+            "method": "%d evaluations at each of %d cutoffs, integrated over the schedule by algorithmic FLOP"
+                      % (reps, samples)}
+
+
+def tri_cpu_baseline(target_s=10.0):
+    """float TRI oracle (reference arithmetic, OpenMP guided over items) on a
+    bounded item sample at five cutoffs, extrapolated by algorithmic FLOP."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import TriOracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    p = build_tri_problem()
+    o = TriOracle(p)
+    v2, v4 = p.initial_state(np.float32)
+    dv2 = np.zeros(p.nw, dtype=np.float32)
+    ncut = len(p.cutoffs)
+    ks = [int(round(x)) for x in np.linspace(0, ncut - 2, 5)]
+    rng = np.random.default_rng(2)
+    n_items = max(threads, 64)
+    for rep in range(2):
+        t_tot, f_tot = 0.0, 0.0
+        for k in ks:
+            start = int(rng.integers(0, p.n_freq - n_items))
+            t0 = time.perf_counter()
+            _, calls = o.items(float(p.cutoffs[k]), v2, dv2, v4, start, start + n_items, dtype=np.float32,
+                               nthreads=threads)
+            t_tot += time.perf_counter() - t0
+            f_tot += p.flops_per_evaluation(*calls.tolist()) - p.flops_per_evaluation(0, 0, 0)
+        if rep == 0:
+            n_items = int(min(p.n_freq // 2, n_items * max(1.0, target_s / max(t_tot, 1e-3))))
+    flow_flops = sum(evaluation_flops(p, p.cutoffs[k]) for k in range(ncut - 1))
+    flow_s = t_tot / f_tot * flow_flops
+    return {"value": (ncut - 1) / flow_s, "unit": "dGamma/dLambda evals/s", "cores": threads, "kind": "port",
+            "sample": "float32 TRI CPU oracle on %d items at each of 5 cutoffs (%.1f s measured, %.3g GFLOP/s); "
+                      "full flow %.0f s extrapolated by algorithmic FLOP" % (n_items, t_tot, f_tot / t_tot / 1e9,
+                                                                          flow_s)}
+
+
+def mode_tri_flow(args):
+    """The whole TRI flow on the GPU, one step = computeStep + correlation +
+    isDiverged + finalizeStep."""
+    p = build_tri_problem()
+    nsteps_flow = len(p.cutoffs) - 1
+    steps = min(args.steps, nsteps_flow)
+    ctx = TriFlowContext(p)
+    v2, v4 = p.initial_state(np.float64)
+    ctx.set_state(float(p.cutoffs[0]), v2, v4)
+    for w in range(args.warmup):
+        ctx.compute_step()
+        ctx.measure_correlation()
+        ctx.finalize_step(float(p.cutoffs[w + 1]))
+    ctx.set_state(float(p.cutoffs[0]), v2, v4)
+    ctx.synchronize()
+    ctx.profile(True)
+    flops = 0.0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.compute_step()
+        ctx.measure_correlation()
+        if ctx.is_diverged():
+            raise RuntimeError("TRI flow diverged at cutoff index %d" % k)
+        ctx.finalize_step(float(p.cutoffs[k + 1]))
+        flops += evaluation_flops(p, p.cutoffs[k])
+        if (k + 1) % 20 == 0:
+            print("# tri step %d / %d  %.1f s" % (k + 1, steps, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    dg_ms, dg_n = ctx.profile_get(KERNEL_DGAMMA)
+    ctx.close()
+    achieved = flops / (dg_ms * 1e-3) / 1e12
+    out = {"metric": "TRI dGamma/dLambda evals/s (full-schedule average, FP64 flow)", "value": steps / elapsed,
+           "unit": "evals/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": elapsed * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (task-file generated flow, initial vertex = model couplings)",
+           "config": {"workload": "kagome-dm-r5-nw48", "L": p.L, "nw": p.nw, "overlap_terms": p.overlap_total,
+                      "cutoffs": len(p.cutoffs), "step": "computeStep + correlation + isDiverged + finalizeStep"},
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_tri_dgamma",
+                        "launches": dg_n, "avg_launch_ms": dg_ms / max(dg_n, 1),
+                        "algorithmic_tflop_total": flops / 1e12},
+           "cpu_baseline": None}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = tri_cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+
+
 def mode_flow(args):
     prob = build_problem(args.workload)
     nsteps_flow = len(prob.cutoffs) - 1
@@ -281,8 +423,12 @@ def mode_flow(args):
         mid = float(prob.cutoffs[(nsteps_flow - 1) // 2])
         mixed["max_rel_err_derivative_initial_state_mid_cutoff"] = derivative_errors(prob, (mid, *init))
         out["mixed"] = mixed
+    if not args.no_tri:
+        out["tri"] = tri_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+        if not args.no_tri:
+            out["tri"]["cpu_baseline"] = tri_cpu_baseline(min(10.0, args.cpu_seconds))
     print(json.dumps(out))
 
 
@@ -362,11 +508,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=655)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="flow", choices=("flow", "sweep"))
+    ap.add_argument("--mode", default="flow", choices=("flow", "sweep", "tri-flow"))
     ap.add_argument("--workload", default="cubic-j1j2-r6-nw64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-fixed", action="store_true")
+    ap.add_argument("--no-tri", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--flows", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2024)
@@ -381,6 +528,8 @@ def main():
         sys.exit(2)
     if args.mode == "flow":
         mode_flow(args)
+    elif args.mode == "tri-flow":
+        mode_tri_flow(args)
     else:
         mode_sweep(args)
 

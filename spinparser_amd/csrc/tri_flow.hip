@@ -453,7 +453,7 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
         } else {
           // getValueLocal (TRIVertexTwoParticle.hpp:300-309): site 0; the sign
           // table is indexed with the exchanged components
-#pragma unroll
+#pragma unroll 4
           for (int cc = 0; cc < 16; ++cc) {
             int s1 = cc / 4, s2 = cc % 4;
             if (exch) { const int q = s1; s1 = s2; s2 = q; }
@@ -482,19 +482,24 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
 #pragma unroll
         for (int r = 0; r < 4; ++r) { off[r] = s_off[d * 4 + r]; w[r] = s_w[d * 4 + r]; }
         double *dst = s_A + (size_t)d * 16 * L + j;
+        // four components (16 loads) in flight per group keeps the register
+        // footprint of the gather small next to the live accumulators
+#pragma unroll 1
+        for (int s1 = 0; s1 < 4; ++s1) {
 #pragma unroll
-        for (int cc = 0; cc < 16; ++cc) {
-          const int s1 = cc / 4, s2 = cc % 4;
-          const double tsign = tu ? double(zeta(s1) * zeta(s2)) : 1.0;
-          const double fsign = -double(zeta(exch ? s1 : s2));
-          const int co = cofs[cc * L + j];
-          double v = 0.0;
+          for (int s2 = 0; s2 < 4; ++s2) {
+            const int cc = 4 * s1 + s2;
+            const double tsign = tu ? double(zeta(s1) * zeta(s2)) : 1.0;
+            const double fsign = -double(zeta(exch ? s1 : s2));
+            const int co = cofs[cc * L + j];
+            double v = 0.0;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const double sg = ((fl >> (2 + r)) & 1) ? tsign * fsign : tsign;
-            v = fma(sg * w[r], gload(rG, (off[r] + co) * 8), v);
+            for (int r = 0; r < 4; ++r) {
+              const double sg = ((fl >> (2 + r)) & 1) ? tsign * fsign : tsign;
+              v = fma(sg * w[r], gload(rG, (off[r] + co) * 8), v);
+            }
+            dst[cc * L] = v;
           }
-          dst[cc * L] = v;
         }
       }
       __syncthreads();
