@@ -1,7 +1,9 @@
 """Build recipe for the HIP library (gfx950) -- compiled in-tree with hipcc.
 
 ``python -m spinparser_amd.build`` (or ``__graft_entry__.build()``) writes
-``spinparser_amd/_lib/libspfrg.so``.  Device code is compiled with
+``spinparser_amd/_lib/libspfrg.so`` (the flow kernels and their C ABI) and
+``spinparser_amd/_lib/libspobs.so`` (host-only HDF5 writer of observable files,
+built when the HDF5 C library is found).  Device code is compiled with
 ``-ffp-contract=off`` so that frequency arguments, window predicates and mesh
 bracket comparisons round exactly like the reference; the hot accumulations use
 explicit ``fma()``.  Packed-FP32 instructions are disabled for all device code.
@@ -18,6 +20,9 @@ ROOT = os.path.dirname(PKG)
 LIB_DIR = os.path.join(PKG, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspfrg.so")
 SOURCES = [os.path.join(PKG, "csrc", "su2_flow.hip"), os.path.join(PKG, "csrc", "tri_flow.hip")]
+OBS_PATH = os.path.join(LIB_DIR, "libspobs.so")
+OBS_SOURCE = os.path.join(PKG, "csrc", "obs_writer.cpp")
+HDF5_PREFIXES = ("/opt/conda", "/usr", "/usr/local")
 
 
 def hipcc() -> str:
@@ -44,5 +49,34 @@ def build_lib(verbose: bool = False) -> str:
     return LIB_PATH
 
 
+def hdf5_prefix():
+    for pre in HDF5_PREFIXES:
+        if os.path.exists(os.path.join(pre, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(pre, "lib", "libhdf5.so")):
+            return pre
+    return None
+
+
+def build_obs(verbose: bool = False):
+    """Host-only HDF5 observable writer; returns its path, or None without HDF5."""
+    pre = hdf5_prefix()
+    if pre is None:
+        return None
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = OBS_PATH + ".tmp"
+    lib = os.path.join(pre, "lib")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(pre, "include"), "-o", tmp, OBS_SOURCE, "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib]
+    if verbose:
+        print(" ".join(cmd))
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed for libspobs.so")
+    os.replace(tmp, OBS_PATH)
+    return OBS_PATH
+
+
 if __name__ == "__main__":
     print(build_lib(verbose=True))
+    print(build_obs(verbose=True))
