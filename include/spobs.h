@@ -39,6 +39,22 @@ int sp_obs_write_correlation(const char *path, const char *group, float cutoff, 
 int sp_obs_count(const char *path, const char *group);
 int sp_obs_read_correlation(const char *path, const char *group, int k, float *cutoff, float *data, int n);
 
+/* Checkpoint files in the reference's data-file layout
+ * (src/SU2/SU2EffectiveAction.hpp:79-205, src/TRI/TRIEffectiveAction.hpp:79-205):
+ *   /checkpoint_<k>  group with float attribute "cutoff" and float datasets
+ *                    "cutoff" (1), "v2" (N_w) and the vertex arrays ("v4dd",
+ *                    "v4ss" for SU(2); "v4" for TRI)
+ * sp_ckpt_write appends (append != 0 and the file exists) or truncates;
+ * returns the checkpoint id written, SP_OBS_DUPLICATE_CKPT (-3) if a
+ * checkpoint at this cutoff exists, or an error code (< 0).  Values are stored
+ * as float, as in the reference.  sp_ckpt_read reads checkpoint `id` (id < 0:
+ * the last one) into the arrays, each of the given size. */
+#define SP_OBS_DUPLICATE_CKPT -3
+int sp_ckpt_write(const char *path, int append, float cutoff, int narrays, const char *const *names,
+                  const long long *sizes, const float *const *data);
+int sp_ckpt_read(const char *path, int id, float *cutoff, int narrays, const char *const *names,
+                 const long long *sizes, float *const *data);
+
 /* Message of the last failure in this thread. */
 const char *sp_obs_last_error(void);
 

@@ -117,7 +117,6 @@ int sp_obs_count(const char *path, const char *group) {
   if (H5Fis_hdf5(path) <= 0) return 0;
   H file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
   if (file.id < 0) return err(SP_OBS_EIO, std::string("could not open ") + path);
-  const std::string dpath = std::string(group) + "/data";
   if (H5Lexists(file.id, group, H5P_DEFAULT) <= 0) return 0;
   H grp(H5Gopen(file.id, group, H5P_DEFAULT), H5Gclose);
   if (H5Lexists(grp.id, "data", H5P_DEFAULT) <= 0) return 0;
@@ -142,6 +141,66 @@ int sp_obs_read_correlation(const char *path, const char *group, int k, float *c
   if (count != n) return err(SP_OBS_EARG, "size mismatch reading " + m);
   if (H5Dread(ds.id, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, data) < 0) return err(SP_OBS_EIO, "read " + m);
   return SP_OBS_OK;
+}
+
+int sp_ckpt_write(const char *path, int append, float cutoff, int narrays, const char *const *names,
+                  const long long *sizes, const float *const *data) {
+  if (!path || narrays < 0 || (narrays > 0 && (!names || !sizes || !data))) return err(SP_OBS_EARG, "invalid argument");
+  H5Eset_auto(H5E_DEFAULT, nullptr, nullptr);
+  H file(append && H5Fis_hdf5(path) > 0 ? H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT)
+                                         : H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT),
+         H5Fclose);
+  if (file.id < 0) return err(SP_OBS_EIO, std::string("could not open data file ") + path);
+  const std::vector<std::string> groups = measurement_names(file.id);
+  for (const std::string &nm : groups) {
+    H g(H5Gopen(file.id, nm.c_str(), H5P_DEFAULT), H5Gclose);
+    H a(H5Aopen(g.id, "cutoff", H5P_DEFAULT), H5Aclose);
+    float c = 0.0f;
+    if (a.id >= 0 && H5Aread(a.id, H5T_NATIVE_FLOAT, &c) >= 0 && c == cutoff) return SP_OBS_DUPLICATE_CKPT;
+  }
+  const int id = (int)groups.size();
+  const std::string name = "checkpoint_" + std::to_string(id);
+  H g(H5Gcreate(file.id, name.c_str(), H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose);
+  if (g.id < 0) return err(SP_OBS_EIO, "could not create " + name);
+  const hsize_t one[1] = {1};
+  H aspace(H5Screate_simple(1, one, nullptr), H5Sclose);
+  H attr(H5Acreate(g.id, "cutoff", H5T_NATIVE_FLOAT, aspace.id, H5P_DEFAULT, H5P_DEFAULT), H5Aclose);
+  if (attr.id < 0 || H5Awrite(attr.id, H5T_NATIVE_FLOAT, &cutoff) < 0) return err(SP_OBS_EIO, "cannot write cutoff");
+  auto dataset = [&](const char *nm, hsize_t n, const float *v) {
+    H space(H5Screate_simple(1, &n, nullptr), H5Sclose);
+    H ds(H5Dcreate(g.id, nm, H5T_NATIVE_FLOAT, space.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+    return ds.id >= 0 && H5Dwrite(ds.id, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, v) >= 0;
+  };
+  if (!dataset("cutoff", 1, &cutoff)) return err(SP_OBS_EIO, "cannot write " + name + "/cutoff");
+  for (int i = 0; i < narrays; ++i)
+    if (!names[i] || !data[i] || sizes[i] < 1 || !dataset(names[i], (hsize_t)sizes[i], data[i]))
+      return err(SP_OBS_EIO, "cannot write " + name + "/" + (names[i] ? names[i] : "?"));
+  return id;
+}
+
+int sp_ckpt_read(const char *path, int id, float *cutoff, int narrays, const char *const *names,
+                 const long long *sizes, float *const *data) {
+  if (!path || !cutoff || narrays < 0 || (narrays > 0 && (!names || !sizes || !data)))
+    return err(SP_OBS_EARG, "invalid argument");
+  H5Eset_auto(H5E_DEFAULT, nullptr, nullptr);
+  H file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+  if (file.id < 0) return err(SP_OBS_EIO, std::string("could not open data file ") + path);
+  if (id < 0) id = (int)measurement_names(file.id).size() - 1;
+  const std::string name = "checkpoint_" + std::to_string(id);
+  H g(H5Gopen(file.id, name.c_str(), H5P_DEFAULT), H5Gclose);
+  if (g.id < 0) return err(SP_OBS_EIO, "no group " + name);
+  auto read = [&](const char *nm, long long n, float *v) {
+    H ds(H5Dopen(g.id, nm, H5P_DEFAULT), H5Dclose);
+    if (ds.id < 0) return false;
+    H space(H5Dget_space(ds.id), H5Sclose);
+    if (H5Sget_simple_extent_npoints(space.id) != n) return false;
+    return H5Dread(ds.id, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, v) >= 0;
+  };
+  if (!read("cutoff", 1, cutoff)) return err(SP_OBS_EIO, "no cutoff in " + name);
+  for (int i = 0; i < narrays; ++i)
+    if (!names[i] || !data[i] || !read(names[i], sizes[i], data[i]))
+      return err(SP_OBS_EIO, "cannot read " + name + "/" + (names[i] ? names[i] : "?"));
+  return id;
 }
 
 }  // extern "C"

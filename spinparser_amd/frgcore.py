@@ -187,6 +187,22 @@ class HipSU2FrgCore(FrgCore):
         self.cutoff = float(cutoff)
         self._state._core_cutoff = self.cutoff
 
+    def writeCheckpoint(self, dataFilePath: str, append: bool = False) -> int:
+        """``SU2EffectiveAction::writeCheckpoint`` (src/SU2/SU2EffectiveAction.hpp:79-150):
+        the resident state as float datasets cutoff, v2, v4dd, v4ss; returns the
+        checkpoint id, or -1 if the file holds one at this cutoff."""
+        from .obs import write_checkpoint
+        _, v2, v4s, v4d = self.context.get_state()
+        return write_checkpoint(dataFilePath, self.cutoff, {"v2": v2, "v4dd": v4d, "v4ss": v4s}, append)
+
+    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1) -> bool:
+        """``SU2EffectiveAction::readCheckpoint`` (:159-205); -1 reads the last checkpoint."""
+        from .obs import read_checkpoint
+        n = self.problem.vertex_size
+        cut, a = read_checkpoint(dataFilePath, {"v2": self.problem.nw, "v4dd": n, "v4ss": n}, checkpointId)
+        self.setState(cut, a["v2"], a["v4ss"], a["v4dd"])
+        return True
+
     def close(self) -> None:
         self.context.close()
 
@@ -226,6 +242,19 @@ class HipTRIFrgCore(FrgCore):
         self.context.set_state(cutoff, v2, v4)
         self.cutoff = float(cutoff)
         self._state._core_cutoff = self.cutoff
+
+    def writeCheckpoint(self, dataFilePath: str, append: bool = False) -> int:
+        """``TRIEffectiveAction::writeCheckpoint`` (src/TRI/TRIEffectiveAction.hpp:79-150)."""
+        from .obs import write_checkpoint
+        _, v2, v4 = self.context.get_state()
+        return write_checkpoint(dataFilePath, self.cutoff, {"v2": v2, "v4": v4}, append)
+
+    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1) -> bool:
+        """``TRIEffectiveAction::readCheckpoint`` (:159-205)."""
+        from .obs import read_checkpoint
+        cut, a = read_checkpoint(dataFilePath, {"v2": self.problem.nw, "v4": self.problem.vertex_size}, checkpointId)
+        self.setState(cut, a["v2"], a["v4"])
+        return True
 
     def close(self) -> None:
         self.context.close()

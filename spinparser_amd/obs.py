@@ -1,4 +1,5 @@
-"""Observable files in the reference's HDF5 layout (ctypes over ``libspobs.so``).
+"""Observable and checkpoint files in the reference's HDF5 layouts (ctypes over
+``libspobs.so``).
 
 ``write_correlation`` appends one measurement of one observable group, exactly
 as ``SU2MeasurementCorrelation::_writeOutfileCorrelation`` /
@@ -8,6 +9,11 @@ as ``SU2MeasurementCorrelation::_writeOutfileCorrelation`` /
 vectors, basis and site positions, then ``data/measurement_<k>`` with the
 float ``cutoff`` attribute and the (basis, sites in range) float data; a
 measurement at a cutoff already in the file is discarded.
+
+``write_checkpoint`` / ``read_checkpoint`` follow ``writeCheckpoint`` /
+``readCheckpoint`` of the effective actions (``src/SU2/SU2EffectiveAction.hpp:79-205``,
+``src/TRI/TRIEffectiveAction.hpp:79-205``): ``checkpoint_<k>`` groups holding
+the cutoff and the float vertex arrays.
 """
 from __future__ import annotations
 
@@ -43,6 +49,12 @@ def load_obs_library() -> ctypes.CDLL:
         lib.sp_obs_read_correlation.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _f32p, _f32p,
                                                 ctypes.c_int]
         lib.sp_obs_last_error.restype = ctypes.c_char_p
+        _cpp = ctypes.POINTER(ctypes.c_char_p)
+        _llp = ctypes.POINTER(ctypes.c_longlong)
+        lib.sp_ckpt_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, _cpp, _llp,
+                                      ctypes.POINTER(_f32p)]
+        lib.sp_ckpt_read.argtypes = [ctypes.c_char_p, ctypes.c_int, _f32p, ctypes.c_int, _cpp, _llp,
+                                     ctypes.POINTER(_f32p)]
         _lib = lib
     return _lib
 
@@ -115,3 +127,44 @@ def write_tri_records(path: str, lattice, records: Iterable, groups: Optional[It
             ok = write_correlation(path, g, r.cutoff, lattice, r.data[g]) and ok
         n += int(ok)
     return n
+
+
+SP_OBS_DUPLICATE_CKPT = -3
+
+
+def _arrays_args(names, arrays):
+    n = len(names)
+    c_names = (ctypes.c_char_p * n)(*[nm.encode() for nm in names])
+    sizes = (ctypes.c_longlong * n)(*[a.size for a in arrays])
+    ptrs = (_f32p * n)(*[_p(a) for a in arrays])
+    return c_names, sizes, ptrs
+
+
+def write_checkpoint(path: str, cutoff: float, arrays: dict, append: bool = True) -> int:
+    """Write one checkpoint (values stored as float); returns its id, or -1 if
+    a checkpoint at this cutoff exists (``EffectiveAction::writeCheckpoint``)."""
+    lib = load_obs_library()
+    names = list(arrays)
+    data = [np.ascontiguousarray(arrays[k], dtype=np.float32).ravel() for k in names]
+    c_names, sizes, ptrs = _arrays_args(names, data)
+    rc = lib.sp_ckpt_write(path.encode(), 1 if append else 0, ctypes.c_float(float(np.float32(cutoff))), len(names),
+                           c_names, sizes, ptrs)
+    if rc == SP_OBS_DUPLICATE_CKPT:
+        return -1
+    if rc < 0:
+        raise ObsError(lib.sp_obs_last_error().decode())
+    return rc
+
+
+def read_checkpoint(path: str, sizes: dict, checkpoint_id: int = -1):
+    """Read checkpoint ``checkpoint_id`` (-1: the last); returns (cutoff, {name:
+    float32 array}) (``EffectiveAction::readCheckpoint``)."""
+    lib = load_obs_library()
+    names = list(sizes)
+    data = [np.zeros(int(sizes[k]), dtype=np.float32) for k in names]
+    c_names, c_sizes, ptrs = _arrays_args(names, data)
+    cut = ctypes.c_float(0)
+    rc = lib.sp_ckpt_read(path.encode(), int(checkpoint_id), ctypes.byref(cut), len(names), c_names, c_sizes, ptrs)
+    if rc < 0:
+        raise ObsError(lib.sp_obs_last_error().decode())
+    return float(cut.value), dict(zip(names, data))

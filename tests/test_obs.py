@@ -91,3 +91,24 @@ def test_writer_exports_header_symbols():
     lib = ctypes.CDLL(obs.OBS_PATH)
     for name in declared:
         assert hasattr(lib, name), name
+
+
+def test_checkpoint_roundtrip_and_duplicates(tmp_path):
+    path = str(tmp_path / "flow.data")
+    rng = np.random.default_rng(3)
+    a = {"v2": rng.standard_normal(14), "v4dd": rng.standard_normal(300), "v4ss": rng.standard_normal(300)}
+    b = {k: v + 1.0 for k, v in a.items()}
+    assert obs.write_checkpoint(path, 10.0, a, append=False) == 0
+    assert obs.write_checkpoint(path, 9.0, b, append=True) == 1
+    assert obs.write_checkpoint(path, 9.0, b, append=True) == -1          # same cutoff: skipped
+    sizes = {k: v.size for k, v in a.items()}
+    cut, got = obs.read_checkpoint(path, sizes)                             # last checkpoint
+    assert cut == np.float32(9.0)
+    for k in a:
+        assert np.array_equal(got[k], b[k].astype(np.float32))
+    cut, got = obs.read_checkpoint(path, sizes, 0)
+    assert cut == np.float32(10.0) and np.array_equal(got["v4ss"], a["v4ss"].astype(np.float32))
+    # a new file without append replaces the old one
+    assert obs.write_checkpoint(path, 5.0, a, append=False) == 0
+    with pytest.raises(obs.ObsError):
+        obs.read_checkpoint(path, sizes, 1)
