@@ -73,7 +73,7 @@ namespace {
 #define SPFRG_MIN_BLOCKS 4
 #endif
 #ifndef SPFRG_MIN_BLOCKS_MIXED
-#define SPFRG_MIN_BLOCKS_MIXED 1
+#define SPFRG_MIN_BLOCKS_MIXED 4
 #endif
 constexpr int kThreads = SPFRG_THREADS;
 constexpr int kMaxBatch = SPFRG_MAX_BATCH;
@@ -148,6 +148,7 @@ struct FlowDev {
   double *v2, *dv2;
   const double *mesh;
   const int *site_rid, *inv_rid, *ovl_off, *ovl_rid1, *ovl_rid2, *range_fwd, *range_inv, *corr_rid;
+  const unsigned short *ovl_idx;  // per overlap entry: rid1 * Lp + rid2 (Lp = L rounded up to even)
   double *Wwp, *Wc;
   int *Wn;
   double *Cwp, *Cw;
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
         double ps = 0.0, pd = 0.0;
         const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
         for (int i = b + lane; i < e; i += 64) {
-          const int idx = F.ovl_rid1[i] * Lp + F.ovl_rid2[i];
+          const int idx = F.ovl_idx[i];
           if (MIXED) {
             KSum a0, a1;
             __builtin_memcpy(&a0, s_C + idx, 8);
@@ -1052,6 +1053,7 @@ int sp_ctx_footprint(const sp_su2_problem *p, long long *bytes) {
   b += 2 * (long long)p->nw * emax * 8 + 2 * emax * 8 + (p->nw + 1) * 4;  // weight tables
   b += emax * p->L * 16 + 2LL * p->B * p->Rb * 8 + 16;                   // correlation scratch / output, flags
   b += 4LL * (3 * p->L + 1 + 2 * (p->ovl_off ? p->ovl_off[p->L] : 0) + 2 * p->R + p->B * p->Rb);
+  b += 2LL * (p->ovl_off ? p->ovl_off[p->L] : 0);                         // packed overlap indices
   b += (long long)sizeof(FlowDev);
   *bytes = b;
   return SP_OK;
@@ -1092,6 +1094,8 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   double *mesh = nullptr;
   int *site = nullptr, *inv = nullptr, *ooff = nullptr, *o1 = nullptr, *o2 = nullptr, *rf = nullptr, *ri = nullptr,
       *corr = nullptr;
+  unsigned short *oidx = nullptr;
+  if ((rc = dalloc(ctx, &oidx, O))) return rc;
   if ((rc = dalloc(ctx, &mesh, f.nw)) || (rc = dalloc(ctx, &site, f.L)) || (rc = dalloc(ctx, &inv, f.L)) ||
       (rc = dalloc(ctx, &ooff, f.L + 1)) || (rc = dalloc(ctx, &o1, O)) || (rc = dalloc(ctx, &o2, O)) ||
       (rc = dalloc(ctx, &rf, f.R)) || (rc = dalloc(ctx, &ri, f.R)) || (rc = dalloc(ctx, &corr, (size_t)f.B * f.Rb)) ||
@@ -1110,6 +1114,7 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   f.ovl_off = ooff;
   f.ovl_rid1 = o1;
   f.ovl_rid2 = o2;
+  f.ovl_idx = oidx;
   f.range_fwd = rf;
   f.range_inv = ri;
   f.corr_rid = corr;
@@ -1123,6 +1128,11 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   if (O > 0) {
     SP_HIP(hipMemcpy(o1, p->ovl_rid1, (size_t)O * 4, hipMemcpyHostToDevice));
     SP_HIP(hipMemcpy(o2, p->ovl_rid2, (size_t)O * 4, hipMemcpyHostToDevice));
+    // packed C-matrix index of every overlap entry (rid < L <= 64, so it fits 16 bits)
+    const int Lp = f.L + (f.L & 1);
+    std::vector<unsigned short> idx(O);
+    for (int i = 0; i < O; ++i) idx[i] = (unsigned short)(p->ovl_rid1[i] * Lp + p->ovl_rid2[i]);
+    SP_HIP(hipMemcpy(oidx, idx.data(), (size_t)O * 2, hipMemcpyHostToDevice));
   }
   SP_HIP(hipMemcpy(rf, p->range_fwd, (size_t)f.R * 4, hipMemcpyHostToDevice));
   SP_HIP(hipMemcpy(ri, p->range_inv, (size_t)f.R * 4, hipMemcpyHostToDevice));
