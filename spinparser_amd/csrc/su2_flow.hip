@@ -86,6 +86,16 @@ constexpr int kMaxBatch = SPFRG_MAX_BATCH;
 #ifndef SPFRG_CHUNK
 #define SPFRG_CHUNK 16
 #endif
+// support gathers of one evaluation point when U == 1: 0 = per access buffer,
+// 1 = the next buffer's four loads in flight while the current one is used,
+// 2 = all sixteen loads issued before any use (the FP32 gathers of the mixed
+// form fit this in the occupancy-4 register budget; FP64 does not)
+#ifndef SPFRG_GATHER
+#define SPFRG_GATHER 0
+#endif
+#ifndef SPFRG_GATHER_MIXED
+#define SPFRG_GATHER_MIXED 2
+#endif
 constexpr long long kChunk = SPFRG_CHUNK;
 
 using namespace frg;
@@ -508,13 +518,15 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
           bool ok[U];
           // U == 1: loads and weights per access buffer, leaving the schedule
           // to the compiler; U > 1: all 16 U loads issued before any use
-          V2 val[U > 1 ? U : 1][16];
+          constexpr int kGather = MIXED ? SPFRG_GATHER_MIXED : SPFRG_GATHER;
+          constexpr bool kPre = U > 1 || kGather == 2;
+          V2 val[kPre ? U : 1][16];
 #pragma unroll
           for (int q = 0; q < U; ++q) {
             ev[q] = e + q * slots;
             ok[q] = ev[q] < nvalid;
             if (!ok[q]) ev[q] = e;
-            if (U > 1) {
+            if (kPre) {
 #pragma unroll
               for (int k = 0; k < 4; ++k) {
                 const int sk = ev[q] * 4 + k;
@@ -532,15 +544,34 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
             if (!ok[q]) continue;
             const int ee = ev[q];
             V As[4], Ad[4];
+            V2 vn[4];
+            if (!kPre && kGather == 1) {
+              const int sr = s_ex[ee * 4] ? sr_inv : sr_fwd;
+              const int4 o = s_off[ee * 4];
+              vn[0] = buf_load<V2>(rG, o.x + sr);
+              vn[1] = buf_load<V2>(rG, o.y + sr);
+              vn[2] = buf_load<V2>(rG, o.z + sr);
+              vn[3] = buf_load<V2>(rG, o.w + sr);
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
               const int sk = ee * 4 + k;
               const V2 ws01 = s_ws[2 * sk], ws23 = s_ws[2 * sk + 1];
               const V2 wd01 = s_wd[2 * sk], wd23 = s_wd[2 * sk + 1];
               V2 v[4];
-              if (U > 1) {
+              if (kPre) {
                 v[0] = val[q][4 * k + 0]; v[1] = val[q][4 * k + 1];
                 v[2] = val[q][4 * k + 2]; v[3] = val[q][4 * k + 3];
+              } else if (kGather == 1) {
+                v[0] = vn[0]; v[1] = vn[1]; v[2] = vn[2]; v[3] = vn[3];
+                if (k < 3) {
+                  const int sr = s_ex[sk + 1] ? sr_inv : sr_fwd;
+                  const int4 o = s_off[sk + 1];
+                  vn[0] = buf_load<V2>(rG, o.x + sr);
+                  vn[1] = buf_load<V2>(rG, o.y + sr);
+                  vn[2] = buf_load<V2>(rG, o.z + sr);
+                  vn[3] = buf_load<V2>(rG, o.w + sr);
+                }
               } else {
                 const int sr = s_ex[sk] ? sr_inv : sr_fwd;
                 const int4 o = s_off[sk];
