@@ -188,7 +188,15 @@ class TriOracle:
                               _ptr(k[7], _i32p), _ptr(k[8], _i32p), _ptr(k[9], _i32p), len(prob.range_fwd),
                               _ptr(k[10], _i32p), _ptr(k[11], _i32p), prob.B, prob.Rb, _ptr(k[12], _i32p),
                               _ptr(k[13], _i32p))
-        self.lib = tri_lib()
+        self.lib = self._load()
+
+    # per-core constants: library, symbol prefix, correlation groups
+    _prefix = "or_tri_"
+    groups = TRI_GROUPS
+
+    @staticmethod
+    def _load():
+        return tri_lib()
 
     def _t(self, dtype):
         return (_f64p, ctypes.c_double, "f64") if dtype == np.float64 else (_f32p, ctypes.c_float, "f32")
@@ -202,7 +210,7 @@ class TriOracle:
         dv2 = np.zeros(p.nw, dtype=dtype)
         dv4 = np.zeros(p.vertex_size, dtype=dtype)
         calls = np.zeros(3, dtype=np.int64)
-        getattr(self.lib, "or_tri_compute_step_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t),
+        getattr(self.lib, self._prefix + "compute_step_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t),
                                                          _ptr(v4, t), _ptr(dv2, t), _ptr(dv4, t),
                                                          ctypes.c_int(nthreads), _ptr(calls, _lp))
         return dv2, dv4, calls
@@ -213,31 +221,32 @@ class TriOracle:
         v2, dv2, v4 = (np.ascontiguousarray(x, dtype=dtype) for x in (v2, dv2, v4))
         dv4 = np.zeros(self.prob.vertex_size, dtype=dtype)
         calls = np.zeros(3, dtype=np.int64)
-        getattr(self.lib, "or_tri_items_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t), _ptr(dv2, t),
+        getattr(self.lib, self._prefix + "items_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t), _ptr(dv2, t),
                                                   _ptr(v4, t), ctypes.c_int(it0), ctypes.c_int(it1), _ptr(dv4, t),
                                                   ctypes.c_int(nthreads), _ptr(calls, _lp))
         return dv4, calls
 
     def correlation(self, cutoff, v2, v4, dtype=np.float64):
-        """dict group -> (B, Rb) array of the 10 TRI correlation outputs."""
+        """dict group -> (B, Rb) array of the correlation outputs (10 for TRI)."""
         p = self.prob
         t, ct, sfx = self._t(dtype)
         v2, v4 = (np.ascontiguousarray(x, dtype=dtype) for x in (v2, v4))
-        out = np.zeros(10 * p.B * p.Rb, dtype=dtype)
-        getattr(self.lib, "or_tri_correlation_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t),
+        ng = len(self.groups)
+        out = np.zeros(ng * p.B * p.Rb, dtype=dtype)
+        getattr(self.lib, self._prefix + "correlation_" + sfx)(ctypes.byref(self.c), ct(float(cutoff)), _ptr(v2, t),
                                                         _ptr(v4, t), _ptr(out, t))
-        out = out.reshape(10, p.B, p.Rb)
-        return {g: out[i] for i, g in enumerate(TRI_GROUPS)}
+        out = out.reshape(ng, p.B, p.Rb)
+        return {g: out[i] for i, g in enumerate(self.groups)}
 
     def run_flow(self, dtype=np.float32, nthreads=None, ncut=None):
-        """Full flow; returns (records[n, 10, B, Rb], n_records, (v2, v4))."""
+        """Full flow; returns (records[n, groups, B, Rb], n_records, (v2, v4))."""
         nthreads = nthreads or os.cpu_count() or 1
         p = self.prob
         cut = np.ascontiguousarray(p.cutoffs if ncut is None else p.cutoffs[:ncut], dtype=np.float32)
         v2, v4 = p.initial_state(dtype)
-        rec = np.zeros((len(cut), 10, p.B, p.Rb), dtype=dtype)
+        rec = np.zeros((len(cut), len(self.groups), p.B, p.Rb), dtype=dtype)
         t, ct, sfx = self._t(dtype)
-        n = getattr(self.lib, "or_tri_run_flow_" + sfx)(ctypes.byref(self.c), ctypes.c_int(len(cut)),
+        n = getattr(self.lib, self._prefix + "run_flow_" + sfx)(ctypes.byref(self.c), ctypes.c_int(len(cut)),
                                                          _ptr(cut, _f32p), _ptr(v2, t), _ptr(v4, t), _ptr(rec, t),
                                                          ctypes.c_int(nthreads))
         return rec, n, (v2, v4)
@@ -248,3 +257,37 @@ class TriOracle:
         out = np.zeros(4096)
         n = tri_lib().or_tri_table(ctypes.c_int(kind), _ptr(out, _f64p))
         return out.reshape(16, 16, 16), n
+
+
+# ---------------------------------------------------------------------------
+# XYZ core (xyz_oracle.cpp)
+# ---------------------------------------------------------------------------
+XYZ_LIB_PATH = os.path.join(HERE, "liboracle_xyz.so")
+XYZ_GROUPS = ("XYZCorXX", "XYZCorYY", "XYZCorZZ", "XYZCorDD")
+
+_xyz_lib = None
+
+
+def xyz_lib():
+    global _xyz_lib
+    if _xyz_lib is None:
+        if not os.path.exists(XYZ_LIB_PATH):
+            build()
+        _xyz_lib = ctypes.CDLL(XYZ_LIB_PATH)
+    return _xyz_lib
+
+
+class XyzOracle(TriOracle):
+    """CPU XYZ flow (4 spin components, state as four planes X, Y, Z, D) on an
+    ``XYZProblem``; same lattice tables and entry points as the TRI oracle."""
+
+    _prefix = "or_xyz_"
+    groups = XYZ_GROUPS
+
+    @staticmethod
+    def _load():
+        return xyz_lib()
+
+    @staticmethod
+    def table(kind):
+        raise NotImplementedError("the XYZ channel algebra has no generated tables")

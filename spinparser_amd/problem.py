@@ -18,7 +18,7 @@ the task file and the lattice factory:
 """
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, fields
 from typing import Dict, Optional
 
 import numpy as np
@@ -280,3 +280,58 @@ def tri_problem_from_lattice(mesh: np.ndarray, cutoffs: np.ndarray, lattice: Lat
 def tri_problem(task: Task, resource_path: Optional[str] = None) -> TRIProblem:
     lattice, model, core_options = task.build_lattice(resource_path)
     return tri_problem_from_lattice(task.frequencies, task.cutoffs, lattice, core_options)
+
+
+# ---------------------------------------------------------------------------
+# XYZ core (diagonal couplings, 4 spin components X, Y, Z, D)
+# ---------------------------------------------------------------------------
+@dataclass
+class XYZProblem(TRIProblem):
+    """Tables of one XYZ flow (``src/XYZ/XYZFrgCore.cpp``): the lattice tables of
+    the TRI core (site and overlap spin permutations act on the diagonal spin
+    components X, Y, Z; the density D is permutation invariant).  The state is
+    four planes X, Y, Z, D of N_f x L values (``_dataXX`` ... ``_dataDD`` of
+    ``src/XYZ/XYZVertexTwoParticle.hpp:63-77``)."""
+
+    components = 4
+
+    @property
+    def vertex_size(self) -> int:
+        return self.n_freq * 4 * self.L
+
+    def initial_state(self, dtype=np.float64):
+        """(v2, v4) at the first cutoff: Gamma_c(rid) = J_cc(0->rid) / (4 norm) for
+        c = X, Y, Z at all frequencies, D = 0 (``src/XYZ/XYZEffectiveAction.hpp:37-62``;
+        off-diagonal couplings are ignored, as by the reference)."""
+        v2 = np.zeros(self.nw, dtype=dtype)
+        src = self.coupling_f32 if dtype == np.float32 else self.coupling_f64
+        planes = np.zeros((4, self.n_freq, self.L), dtype=dtype)
+        for c in range(3):
+            planes[c] = src[:, c, c][None, :]
+        return v2, planes.ravel()
+
+    def flops_per_evaluation(self, calls_s: int, calls_t: int, calls_u: int) -> float:
+        """Algorithmic FLOP of one dGamma evaluation from integrand-call counts.
+        Per integrand call: 4 bundled gathers of 4 L values (2 FLOP per support
+        x 4 supports: 128 L); S, U: 2 pairs x 16 products (64 L); T: chalice and
+        inverse chalice 2 pairs x 2 x 16 scalar products (128 L), their 4 local
+        vertices of 4 components (4 x 4 x 8 FLOP), and the lattice bubble of one
+        pair (4 products per overlap entry, 8 O) scaled into the result (8 L).
+        Euler 2 |Gamma|."""
+        L, O = self.L, self.overlap_total
+        c_su = 128.0 * L + 64.0 * L
+        c_t = 128.0 * L + 128.0 * L + 128.0 + 8.0 * O + 8.0 * L
+        return c_su * (calls_s + calls_u) + c_t * calls_t + 2.0 * self.vertex_size + 2.0 * self.nw
+
+
+def xyz_problem_from_lattice(mesh: np.ndarray, cutoffs: np.ndarray, lattice: Lattice,
+                             core_options: Dict[str, str]) -> XYZProblem:
+    """XYZ tables; core options as the reference's XYZFrgCore constructor
+    (``src/XYZ/XYZFrgCore.cpp:17-29``: ``normalization``, default 1)."""
+    t = tri_problem_from_lattice(mesh, cutoffs, lattice, core_options)
+    return XYZProblem(**{f.name: getattr(t, f.name) for f in fields(TRIProblem)})
+
+
+def xyz_problem(task: Task, resource_path: Optional[str] = None) -> XYZProblem:
+    lattice, model, core_options = task.build_lattice(resource_path)
+    return xyz_problem_from_lattice(task.frequencies, task.cutoffs, lattice, core_options)

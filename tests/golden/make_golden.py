@@ -7,6 +7,11 @@ and need neither at test time:
 * ``reference1_su2.npz``  -- every dataset and ``cutoff`` attribute of the SU2
   golden flow ``test/scripted/assets/test_reference1.ref`` (hyperkagome range 3,
   Heisenberg J=1, 14 manual frequencies, cutoff 10 -> 0.3 x0.9), float32;
+* ``reference2_xyz.npz``  -- the same for the XYZ golden flow
+  ``test_reference2.ref`` (hyperhoneycomb range 3, Heisenberg-Kitaev j=0.1,
+  k=-1, same frequencies and cutoffs, task in ``test_reference2.sh``);
+* ``reference3_tri.npz``  -- the same for the TRI golden flow
+  ``test_reference3.ref`` (honeycomb Kitaev-Gamma range 3);
 * ``python_obs_su2.json`` -- the literal honeycomb SU2 flow values held in
   ``test/scripted/assets/test_python_obs.py`` (task in ``test_pythonObs.sh``);
 * ``res/*.xml``           -- the unit cells and models those two flows use, cut
@@ -107,11 +112,14 @@ def main():
     assets = os.path.join(ref, "test/scripted/assets")
     n = dump_ref(os.path.join(assets, "test_reference1.ref"), os.path.join(HERE, "reference1_su2.npz"))
     print("reference1_su2.npz:", n, "arrays")
+    n = dump_ref(os.path.join(assets, "test_reference2.ref"), os.path.join(HERE, "reference2_xyz.npz"))
+    print("reference2_xyz.npz:", n, "arrays")
     n = dump_ref(os.path.join(assets, "test_reference3.ref"), os.path.join(HERE, "reference3_tri.npz"))
     print("reference3_tri.npz:", n, "arrays")
     dump_python_obs(ref, os.path.join(HERE, "python_obs_su2.json"))
-    cut_resources(ref, os.path.join(HERE, "res"), ["hyperkagome", "honeycomb"],
-                  ["hyperkagome-heisenberg", "honeycomb-heisenberg", "honeycomb-kitaev-gamma"])
+    cut_resources(ref, os.path.join(HERE, "res"), ["hyperkagome", "honeycomb", "hyperhoneycomb"],
+                  ["hyperkagome-heisenberg", "honeycomb-heisenberg", "honeycomb-kitaev-gamma",
+                   "hyperhoneycomb-kitaev"])
     print("fixtures written to", HERE)
 
 

@@ -86,3 +86,13 @@ def cfg4_problem(nw=48, lattice_range=5):
     """Kagome J=1 with out-of-plane DM D=0.2, range 5, log mesh [0.005, 50], cutoff 50 -> 0.01 x0.98."""
     return tri_problem_of("kagome", lattice_range, "kagome-DM", {"j": 1.0, "d": 0.2}, nw, 0.005, 50.0, 50.0, 0.01,
                           0.98)
+
+
+def xyz_ref2_problem(symmetry="XYZ"):
+    """Hyperhoneycomb range 3, Heisenberg-Kitaev j=0.1 k=-1 (test/scripted/test_reference2.sh:17-53);
+    symmetry "TRI" gives the TRI tables of the same task (the reference's cross-core check)."""
+    from spinparser_amd.problem import tri_problem, xyz_problem
+    xml = manual_task_xml("hyperhoneycomb", 3, "hyperhoneycomb-kitaev", {"j": "0.1", "k": "-1.0"},
+                          symmetry=symmetry)
+    build = xyz_problem if symmetry == "XYZ" else tri_problem
+    return build(parse_task(xml, True), GOLDEN_RES)
