@@ -206,6 +206,47 @@ int sp_tri_profile_get(sp_tri_ctx *ctx, int kernel, double *total_ms, long *laun
 /* Device bytes held by a TRI context. */
 int sp_tri_ctx_footprint(const sp_tri_problem *problem, long long *bytes);
 
+/* ------------------------------------------------------------------------
+ * XYZ core: diagonal two-spin couplings J_xx, J_yy, J_zz, four vertex
+ * components X, Y, Z, D (src/XYZ/XYZFrgCore.cpp).  FP64.  The lattice tables
+ * are those of the TRI core (the spin permutations act on X, Y, Z).
+ *
+ * Vertex layout (src/XYZ/XYZVertexTwoParticle.hpp:63-77,606-640): four planes
+ * X, Y, Z, D (_dataXX, _dataYY, _dataZZ, _dataDD) of N_f * L values, plane c at
+ * c * N_f * L, entry ((s(s+1)/2 + u) * Nw + t) * L + rid, s >= u; Sigma[Nw].
+ * ---------------------------------------------------------------------- */
+typedef sp_tri_problem sp_xyz_problem;
+typedef struct sp_xyz_ctx sp_xyz_ctx;
+
+/* Construction of XYZFrgCore + XYZEffectiveAction (src/XYZ/XYZFrgCore.cpp:17-98). */
+int sp_xyz_ctx_create(const sp_xyz_problem *problem, sp_xyz_ctx **out);
+/* XYZFrgCore::~XYZFrgCore (src/XYZ/XYZFrgCore.cpp:100-104). */
+int sp_xyz_ctx_destroy(sp_xyz_ctx *ctx);
+/* State upload: the initial condition / readCheckpoint of XYZEffectiveAction
+ * (src/XYZ/XYZEffectiveAction.hpp:38-66,164-217). */
+int sp_xyz_set_state(sp_xyz_ctx *ctx, double cutoff, const double *v2, const double *v4);
+/* State download (writeCheckpoint's read side, src/XYZ/XYZEffectiveAction.hpp:82-155). */
+int sp_xyz_get_state(sp_xyz_ctx *ctx, double *cutoff, double *v2, double *v4);
+/* FrgCore::_flow of the last step (src/FrgCore.hpp:140). */
+int sp_xyz_get_flow(sp_xyz_ctx *ctx, double *dv2, double *dv4);
+/* XYZFrgCore::computeStep (src/XYZ/XYZFrgCore.cpp:106-126,164-571).  Asynchronous. */
+int sp_xyz_compute_step(sp_xyz_ctx *ctx);
+/* XYZFrgCore::finalizeStep (src/XYZ/XYZFrgCore.cpp:128-162).  Asynchronous. */
+int sp_xyz_finalize_step(sp_xyz_ctx *ctx, double new_cutoff);
+/* XYZEffectiveAction::isDiverged on the flow (src/XYZ/XYZEffectiveAction.hpp:219-238). */
+int sp_xyz_is_diverged(sp_xyz_ctx *ctx, int *diverged);
+/* XYZMeasurementCorrelation::_calculateCorrelation
+ * (src/XYZ/XYZMeasurementCorrelation.cpp:98-212): 4*B*Rb values, the groups
+ * XYZCorXX, YY, ZZ, DD in this order, each in the .obs site order.  Synchronous. */
+int sp_xyz_measure_correlation(sp_xyz_ctx *ctx, double *out);
+int sp_xyz_synchronize(sp_xyz_ctx *ctx);
+const char *sp_xyz_last_error(const sp_xyz_ctx *ctx);
+/* Kernel timing (kernel ids as for the SU(2) core). */
+int sp_xyz_profile_enable(sp_xyz_ctx *ctx, int enable);
+int sp_xyz_profile_get(sp_xyz_ctx *ctx, int kernel, double *total_ms, long *launches);
+/* Device bytes held by an XYZ context. */
+int sp_xyz_ctx_footprint(const sp_xyz_problem *problem, long long *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB_DIR = os.path.join(PKG, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspfrg.so")
-SOURCES = [os.path.join(PKG, "csrc", "su2_flow.hip"), os.path.join(PKG, "csrc", "tri_flow.hip")]
+SOURCES = [os.path.join(PKG, "csrc", f) for f in ("su2_flow.hip", "tri_flow.hip", "xyz_flow.hip")]
 OBS_PATH = os.path.join(LIB_DIR, "libspobs.so")
 OBS_SOURCE = os.path.join(PKG, "csrc", "obs_writer.cpp")
 HDF5_PREFIXES = ("/opt/conda", "/usr", "/usr/local")

@@ -84,6 +84,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.sp_group_last_error.argtypes = [c_grp]
     lib.sp_group_last_error.restype = ctypes.c_char_p
     _declare_tri(lib)
+    _declare_xyz(lib)
     if path is None:
         _lib = lib
     return lib
@@ -324,26 +325,32 @@ def tri_footprint_bytes(prob) -> int:
 class TriFlowContext:
     """One resident TRI flow on the GPU (owner of an ``sp_tri_ctx``)."""
 
+    _api = "sp_tri_"
+    groups = TRI_GROUPS
+
+    def _f(self, name):
+        return getattr(self.lib, self._api + name)
+
     def __init__(self, prob, lib: Optional[ctypes.CDLL] = None):
         self.lib = lib or load_library()
         self.prob = prob
         self._buf = TriProblemBuffers(prob)
         self.ctx = ctypes.c_void_p()
-        rc = self.lib.sp_tri_ctx_create(ctypes.byref(self._buf.struct), ctypes.byref(self.ctx))
+        rc = self._f("ctx_create")(ctypes.byref(self._buf.struct), ctypes.byref(self.ctx))
         if rc != SP_OK:
-            msg = self.lib.sp_tri_last_error(self.ctx).decode() if self.ctx else "sp_tri_ctx_create failed"
+            msg = self._f("last_error")(self.ctx).decode() if self.ctx else self._api + "ctx_create failed"
             if self.ctx:
-                self.lib.sp_tri_ctx_destroy(self.ctx)
+                self._f("ctx_destroy")(self.ctx)
                 self.ctx = ctypes.c_void_p()
             raise FlowError(rc, msg)
 
     def _check(self, rc: int, what: str):
         if rc != SP_OK:
-            raise FlowError(rc, "%s: %s" % (what, self.lib.sp_tri_last_error(self.ctx).decode()))
+            raise FlowError(rc, "%s: %s" % (what, self._f("last_error")(self.ctx).decode()))
 
     def close(self):
         if self.ctx:
-            self.lib.sp_tri_ctx_destroy(self.ctx)
+            self._f("ctx_destroy")(self.ctx)
             self.ctx = ctypes.c_void_p()
 
     def __del__(self):
@@ -356,49 +363,83 @@ class TriFlowContext:
         v2, v4 = (np.ascontiguousarray(x, dtype=np.float64) for x in (v2, v4))
         if v2.shape != (self.prob.nw,) or v4.shape != (self.prob.vertex_size,):
             raise FlowError(SP_EARG, "state arrays have the wrong size")
-        self._check(self.lib.sp_tri_set_state(self.ctx, float(cutoff), _p(v2, _f64p), _p(v4, _f64p)), "sp_tri_set_state")
+        self._check(self._f("set_state")(self.ctx, float(cutoff), _p(v2, _f64p), _p(v4, _f64p)), self._api + "set_state")
 
     def get_state(self):
         c = ctypes.c_double(0)
         v2 = np.zeros(self.prob.nw)
         v4 = np.zeros(self.prob.vertex_size)
-        self._check(self.lib.sp_tri_get_state(self.ctx, ctypes.byref(c), _p(v2, _f64p), _p(v4, _f64p)),
-                    "sp_tri_get_state")
+        self._check(self._f("get_state")(self.ctx, ctypes.byref(c), _p(v2, _f64p), _p(v4, _f64p)),
+                    self._api + "get_state")
         return c.value, v2, v4
 
     def get_flow(self):
         dv2 = np.zeros(self.prob.nw)
         dv4 = np.zeros(self.prob.vertex_size)
-        self._check(self.lib.sp_tri_get_flow(self.ctx, _p(dv2, _f64p), _p(dv4, _f64p)), "sp_tri_get_flow")
+        self._check(self._f("get_flow")(self.ctx, _p(dv2, _f64p), _p(dv4, _f64p)), self._api + "get_flow")
         return dv2, dv4
 
     def compute_step(self):
-        self._check(self.lib.sp_tri_compute_step(self.ctx), "sp_tri_compute_step")
+        self._check(self._f("compute_step")(self.ctx), self._api + "compute_step")
 
     def finalize_step(self, new_cutoff: float):
-        self._check(self.lib.sp_tri_finalize_step(self.ctx, float(new_cutoff)), "sp_tri_finalize_step")
+        self._check(self._f("finalize_step")(self.ctx, float(new_cutoff)), self._api + "finalize_step")
 
     def is_diverged(self) -> bool:
         d = ctypes.c_int(0)
-        self._check(self.lib.sp_tri_is_diverged(self.ctx, ctypes.byref(d)), "sp_tri_is_diverged")
+        self._check(self._f("is_diverged")(self.ctx, ctypes.byref(d)), self._api + "is_diverged")
         return bool(d.value)
 
     def measure_correlation(self):
-        """(10, B, Rb) array in the group order of TRI_GROUPS."""
-        n = 10 * self.prob.B * self.prob.Rb
+        """(groups, B, Rb) array in the order of ``groups`` (TRI_GROUPS: 10)."""
+        ng = len(self.groups)
+        n = ng * self.prob.B * self.prob.Rb
         out = np.zeros(n)
-        self._check(self.lib.sp_tri_measure_correlation(self.ctx, _p(out, _f64p)), "sp_tri_measure_correlation")
-        return out.reshape(10, self.prob.B, self.prob.Rb)
+        self._check(self._f("measure_correlation")(self.ctx, _p(out, _f64p)), self._api + "measure_correlation")
+        return out.reshape(ng, self.prob.B, self.prob.Rb)
 
     def synchronize(self):
-        self._check(self.lib.sp_tri_synchronize(self.ctx), "sp_tri_synchronize")
+        self._check(self._f("synchronize")(self.ctx), self._api + "synchronize")
 
     def profile(self, enable: bool):
-        self._check(self.lib.sp_tri_profile_enable(self.ctx, 1 if enable else 0), "sp_tri_profile_enable")
+        self._check(self._f("profile_enable")(self.ctx, 1 if enable else 0), self._api + "profile_enable")
 
     def profile_get(self, kernel: int) -> Tuple[float, int]:
         ms = ctypes.c_double(0)
         n = ctypes.c_long(0)
-        self._check(self.lib.sp_tri_profile_get(self.ctx, kernel, ctypes.byref(ms), ctypes.byref(n)),
-                    "sp_tri_profile_get")
+        self._check(self._f("profile_get")(self.ctx, kernel, ctypes.byref(ms), ctypes.byref(n)),
+                    self._api + "profile_get")
         return ms.value, n.value
+
+
+# ---------------------------------------------------------------------------
+# XYZ core (sp_xyz_*): the TRI core's lattice tables and entry points with
+# four vertex components stored as the reference's planes X, Y, Z, D
+# ---------------------------------------------------------------------------
+XYZ_EXPORTED_SYMBOLS = tuple(n.replace("sp_tri_", "sp_xyz_") for n in TRI_EXPORTED_SYMBOLS)
+XYZ_GROUPS = ("XYZCorXX", "XYZCorYY", "XYZCorZZ", "XYZCorDD")
+
+
+def _declare_xyz(lib):
+    for name in TRI_EXPORTED_SYMBOLS:
+        src, dst = getattr(lib, name), getattr(lib, name.replace("sp_tri_", "sp_xyz_"))
+        dst.argtypes = src.argtypes
+        dst.restype = src.restype
+
+
+def xyz_footprint_bytes(prob) -> int:
+    lib = load_library()
+    buf = TriProblemBuffers(prob)
+    out = ctypes.c_longlong(0)
+    rc = lib.sp_xyz_ctx_footprint(ctypes.byref(buf.struct), ctypes.byref(out))
+    if rc != SP_OK:
+        raise FlowError(rc, "sp_xyz_ctx_footprint failed")
+    return int(out.value)
+
+
+class XyzFlowContext(TriFlowContext):
+    """One resident XYZ flow on the GPU (owner of an ``sp_xyz_ctx``); state
+    arrays are the four planes X, Y, Z, D of ``XYZProblem.vertex_size``."""
+
+    _api = "sp_xyz_"
+    groups = XYZ_GROUPS

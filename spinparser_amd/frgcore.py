@@ -9,8 +9,9 @@ same way ``SpinParser::runCore`` drives it:
   ``finalizeStep(newCutoff)``, ``takeMeasurements()``, ``flowingFunctional()``,
   ``flow()``;
 * ``FrgCoreFactory.newFrgCore(identifier, ...)`` (``src/FrgCoreFactory.cpp:25-52``)
-  selects the GPU core for ``symmetry="SU2"`` or ``"TRI"`` when the model carries
-  the option ``<backend>hip</backend>`` (or when ``backend="hip"`` is passed);
+  selects the GPU core for ``symmetry="SU2"``, ``"XYZ"`` or ``"TRI"`` when the
+  model carries the option ``<backend>hip</backend>`` (or when ``backend="hip"``
+  is passed);
 * ``run_core`` restates the cutoff loop of ``src/SpinParser.cpp:126-186``.
 """
 from __future__ import annotations
@@ -20,8 +21,8 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from .capi import TRI_GROUPS, FlowContext, SP_PRECISION_FP64, TriFlowContext
-from .problem import SU2Problem, TRIProblem, su2_problem, tri_problem
+from .capi import FlowContext, SP_PRECISION_FP64, TriFlowContext, XyzFlowContext
+from .problem import SU2Problem, TRIProblem, su2_problem, tri_problem, xyz_problem
 from .task import MeasurementSpec, Task
 
 
@@ -55,7 +56,8 @@ class EffectiveAction:
 
     def arrays(self):
         """Download the vertex arrays in the reference layout: (v2, v4ss, v4dd)
-        for the SU(2) core, (v2, v4) for the TRI core."""
+        for the SU(2) core, (v2, v4) for the TRI and XYZ cores (XYZ: the four
+        planes X, Y, Z, D concatenated)."""
         if self._which == "state":
             return tuple(self._ctx.get_state()[1:])
         return self._ctx.get_flow()
@@ -125,7 +127,14 @@ class TRIMeasurementCorrelation(Measurement):
             return
         out = core.context.measure_correlation()
         if isMasterTask:
-            self.records.append(TRICorrelationRecord(cutoff, {g: out[i] for i, g in enumerate(TRI_GROUPS)}))
+            groups = core.context.groups
+            self.records.append(TRICorrelationRecord(cutoff, {g: out[i] for i, g in enumerate(groups)}))
+
+
+class XYZMeasurementCorrelation(TRIMeasurementCorrelation):
+    """GPU correlation measurement of the XYZ core
+    (``src/XYZ/XYZMeasurementCorrelation.cpp:85-212``): the groups ``XYZCorXX``,
+    ``XYZCorYY``, ``XYZCorZZ``, ``XYZCorDD`` (records keyed by group name)."""
 
 
 class FrgCore:
@@ -211,11 +220,13 @@ class HipTRIFrgCore(FrgCore):
     """TRI core on the GPU (``src/TRI/TRIFrgCore.cpp``): 16-component vertex
     resident in HBM, FP64."""
 
+    _context_type = TriFlowContext
+
     def __init__(self, problem: TRIProblem, measurements: List[Measurement]):
         super().__init__(measurements)
         self.problem = problem
         self.normalization = float(problem.normalization)
-        self.context = TriFlowContext(problem)
+        self.context = self._context_type(problem)
         self.cutoff = float(problem.cutoffs[0])
         v2, v4 = problem.initial_state(np.float64)
         self.context.set_state(self.cutoff, v2, v4)
@@ -260,6 +271,32 @@ class HipTRIFrgCore(FrgCore):
         self.context.close()
 
 
+class HipXYZFrgCore(HipTRIFrgCore):
+    """XYZ core on the GPU (``src/XYZ/XYZFrgCore.cpp``): four-component vertex
+    (planes X, Y, Z, D) resident in HBM, FP64."""
+
+    _context_type = XyzFlowContext
+    _planes = ("v4xx", "v4yy", "v4zz", "v4dd")
+
+    def writeCheckpoint(self, dataFilePath: str, append: bool = False) -> int:
+        """``XYZEffectiveAction::writeCheckpoint`` (src/XYZ/XYZEffectiveAction.hpp:82-155):
+        datasets cutoff, v2, v4dd, v4xx, v4yy, v4zz."""
+        from .obs import write_checkpoint
+        _, v2, v4 = self.context.get_state()
+        planes = v4.reshape(4, -1)
+        arrays = {"v2": v2, "v4dd": planes[3], "v4xx": planes[0], "v4yy": planes[1], "v4zz": planes[2]}
+        return write_checkpoint(dataFilePath, self.cutoff, arrays, append)
+
+    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1) -> bool:
+        """``XYZEffectiveAction::readCheckpoint`` (:164-217)."""
+        from .obs import read_checkpoint
+        n = self.problem.vertex_size // 4
+        sizes = {"v2": self.problem.nw, "v4dd": n, "v4xx": n, "v4yy": n, "v4zz": n}
+        cut, a = read_checkpoint(dataFilePath, sizes, checkpointId)
+        self.setState(cut, a["v2"], np.concatenate([a[k] for k in self._planes]))
+        return True
+
+
 class FrgCoreFactory:
     @staticmethod
     def newFrgCore(identifier: str, problem, measurements: List[MeasurementSpec],
@@ -274,12 +311,16 @@ class FrgCoreFactory:
                 objs.append(SU2MeasurementCorrelation(spec))
             elif identifier == "TRI":
                 objs.append(TRIMeasurementCorrelation(spec))
+            elif identifier == "XYZ":
+                objs.append(XYZMeasurementCorrelation(spec))
             else:
                 raise InitializationError("Measurement [correlation]: no GPU measurement for '%s'." % identifier)
         if identifier == "SU2" and backend == "hip":
             return HipSU2FrgCore(problem, objs)
         if identifier == "TRI" and backend == "hip":
             return HipTRIFrgCore(problem, objs)
+        if identifier == "XYZ" and backend == "hip":
+            return HipXYZFrgCore(problem, objs)
         if identifier in ("SU2", "XYZ", "TRI"):
             raise ArgumentError("Core '%s' with backend '%s' is not provided by this engine" % (identifier, backend))
         raise ArgumentError("Spin model identifier '%s' does not exist." % identifier)
@@ -288,6 +329,8 @@ class FrgCoreFactory:
 def new_core_from_task(task: Task, resource_path: Optional[str] = None, backend: str = "hip") -> FrgCore:
     if task.symmetry == "TRI":
         prob = tri_problem(task, resource_path)
+    elif task.symmetry == "XYZ":
+        prob = xyz_problem(task, resource_path)
     else:
         prob = su2_problem(task, resource_path)
     options = {k: v for k, v in task.model_options.items() if k in ("spin", "normalization", "backend")}

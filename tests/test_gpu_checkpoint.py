@@ -7,7 +7,7 @@ float rounding (the reference's own tolerance, 1e-5 absolute, is the bar).
 import numpy as np
 import pytest
 
-from helpers import ref1_problem, tri_ref3_problem
+from helpers import ref1_problem, tri_ref3_problem, xyz_ref2_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -26,11 +26,12 @@ def _flat(r):
     return np.concatenate([np.ravel(x) for x in r]) if isinstance(r, tuple) else np.ravel(r)
 
 
-@pytest.mark.parametrize("core_kind", ["SU2", "TRI"])
+@pytest.mark.parametrize("core_kind", ["SU2", "TRI", "XYZ"])
 def test_resume_from_checkpoint(tmp_path, core_kind):
-    from spinparser_amd.frgcore import HipSU2FrgCore, HipTRIFrgCore
-    p = ref1_problem() if core_kind == "SU2" else tri_ref3_problem()
-    make = (lambda: HipSU2FrgCore(p, [])) if core_kind == "SU2" else (lambda: HipTRIFrgCore(p, []))
+    from spinparser_amd.frgcore import HipSU2FrgCore, HipTRIFrgCore, HipXYZFrgCore
+    p = {"SU2": ref1_problem, "TRI": tri_ref3_problem, "XYZ": xyz_ref2_problem}[core_kind]()
+    cls = {"SU2": HipSU2FrgCore, "TRI": HipTRIFrgCore, "XYZ": HipXYZFrgCore}[core_kind]
+    make = lambda: cls(p, [])  # noqa: E731
     path = str(tmp_path / "flow.data")
     n = len(p.cutoffs) - 1
     mid = n // 2

@@ -146,11 +146,11 @@ def test_work_counter_matches_oracle():
 def test_library_exports_header_symbols():
     """libspfrg.so loads without a GPU and exports every function of include/spfrg.h."""
     from spinparser_amd.build import LIB_PATH
-    from spinparser_amd.capi import EXPORTED_SYMBOLS, TRI_EXPORTED_SYMBOLS
+    from spinparser_amd.capi import EXPORTED_SYMBOLS, TRI_EXPORTED_SYMBOLS, XYZ_EXPORTED_SYMBOLS
     with open(os.path.join(ROOT, "include", "spfrg.h")) as fh:
         header = fh.read()
     declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(sp_\w+)\s*\(", header, flags=re.M))
-    assert declared == set(EXPORTED_SYMBOLS) | set(TRI_EXPORTED_SYMBOLS)
+    assert declared == set(EXPORTED_SYMBOLS) | set(TRI_EXPORTED_SYMBOLS) | set(XYZ_EXPORTED_SYMBOLS)
     lib = ctypes.CDLL(LIB_PATH)
     for name in declared:
         assert hasattr(lib, name), name
