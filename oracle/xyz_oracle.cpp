@@ -12,11 +12,11 @@
 //                                                   buffers, density sign, site spin
 //                                                   permutations
 //   src/XYZ/XYZVertexSingleParticle.hpp:73-87       self-energy interpolation
-//   src/XYZ/XYZFrgCore.cpp:127-185                  finalizeStep, dSigma
-//   src/XYZ/XYZFrgCore.cpp:187-570                  dGamma: S, T (lattice bubble, chalice,
+//   src/XYZ/XYZFrgCore.cpp:128-193                  finalizeStep, dSigma
+//   src/XYZ/XYZFrgCore.cpp:195-571                  dGamma: S, T (lattice bubble, chalice,
 //                                                   inverse chalice), U, conventional, Katanin
-//   src/XYZ/XYZMeasurementCorrelation.cpp:97-200    static susceptibility, 4 outputs
-//   src/XYZ/XYZEffectiveAction.hpp:37-62            initial condition
+//   src/XYZ/XYZMeasurementCorrelation.cpp:98-212    static susceptibility, 4 outputs
+//   src/XYZ/XYZEffectiveAction.hpp:38-66            initial condition
 //
 // State layout (the reference's): v4 holds four planes X, Y, Z, D of N_f * L
 // values each, plane c at v4 + c * N_f * L, entry ((s(s+1)/2 + u) * N_w + t) * L + rid.
@@ -104,7 +104,7 @@ template <class T> struct XyzCtx {
     return sign * ((1 - bias) * data[lo] + bias * data[hi]);
   }
 
-  // _generateAccessBufferOffset (XYZVertexTwoParticle.hpp:616-632)
+  // _generateAccessBufferOffset (XYZVertexTwoParticle.hpp:637-653)
   int support(int so, int to, int uo, int &sgn) const {
     sgn = so < uo ? -1 : 1;
     if (so < uo) std::swap(so, uo);
@@ -287,7 +287,7 @@ template <class T> inline void scal_acc(T *r, const T *x, T a, int sg, int L) {
 constexpr int kChal[3][2] = {{Y, Z}, {X, Z}, {X, Y}};
 constexpr int kIChal[3][2] = {{Z, Y}, {Z, X}, {X, Y}};
 
-// XYZFrgCore::_calculateVertexTwoParticle (src/XYZ/XYZFrgCore.cpp:187-570)
+// XYZFrgCore::_calculateVertexTwoParticle (src/XYZ/XYZFrgCore.cpp:195-571)
 template <class T>
 void xyz_vertex_two_particle(const XyzCtx<T> &c, int it, XyzScratch<T> &sc, T cutoff, T *out, long *calls) {
   const int L = c.L, nw = c.nw;
@@ -386,7 +386,7 @@ void xyz_vertex_two_particle(const XyzCtx<T> &c, int it, XyzScratch<T> &sc, T cu
   Bundle4<T> &cur = sc.cur;
   cur.reset();
   long nS = 0, nT = 0, nU = 0;
-  // conventional contribution (XYZFrgCore.cpp:461-482)
+  // conventional contribution (XYZFrgCore.cpp:488-509)
   kS(cutoff, sc.b1); ++nS; cur.mult_add(p(cutoff, cutoff + s), sc.b1);
   if (s > T(2.0) * cutoff) { kS(-cutoff, sc.b1); ++nS; cur.mult_add(p(cutoff, cutoff - s), sc.b1); }
   kT(cutoff, sc.b1); ++nT; cur.mult_add(p(cutoff, cutoff + t), sc.b1);
@@ -394,7 +394,7 @@ void xyz_vertex_two_particle(const XyzCtx<T> &c, int it, XyzScratch<T> &sc, T cu
   kU(cutoff, sc.b1); ++nU; cur.mult_add(p(cutoff, cutoff + u), sc.b1);
   if (u > T(2.0) * cutoff) { kU(-cutoff, sc.b1); ++nU; cur.mult_add(p(cutoff, cutoff - u), sc.b1); }
 
-  // Katanin contribution (XYZFrgCore.cpp:484-535)
+  // Katanin contribution (XYZFrgCore.cpp:511-562)
   auto fS = [&](T wp, Bundle4<T> &r) { kS(wp, r); r.scale(pK(wp, s + wp)); };
   auto fT = [&](T wp, Bundle4<T> &r) { kT(wp, r); r.scale(pK(wp, t + wp)); };
   auto fU = [&](T wp, Bundle4<T> &r) { kU(wp, r); r.scale(pK(wp, u + wp)); };
@@ -407,7 +407,7 @@ void xyz_vertex_two_particle(const XyzCtx<T> &c, int it, XyzScratch<T> &sc, T cu
   if (calls) { calls[0] += nS; calls[1] += nT; calls[2] += nU; }
 }
 
-// XYZFrgCore::_calculateVertexSingleParticle (src/XYZ/XYZFrgCore.cpp:152-185)
+// XYZFrgCore::_calculateVertexSingleParticle (src/XYZ/XYZFrgCore.cpp:164-193)
 template <class T> T xyz_vertex_single_particle(const XyzCtx<T> &c, int i, T cutoff) {
   const T w = c.mesh.at(i);
   T v = 0;
@@ -451,7 +451,7 @@ void xyz_compute_step(const XyzProblem &p, T cutoff, const T *v2, const T *v4, T
   xyz_items<T>(p, cutoff, v2, dv2, v4, 0, nf, dv4, nthreads, calls);
 }
 
-// XYZMeasurementCorrelation::_calculateCorrelation (src/XYZ/XYZMeasurementCorrelation.cpp:97-200).
+// XYZMeasurementCorrelation::_calculateCorrelation (src/XYZ/XYZMeasurementCorrelation.cpp:98-212).
 // Output: 4 groups of B*Rb values in the order XX, YY, ZZ, DD.
 template <class T> void xyz_correlation(const XyzProblem &p, T cut, const T *v2, const T *v4, T *out) {
   XyzCtx<T> c(p, v2, nullptr, v4);
@@ -525,7 +525,7 @@ int xyz_run_flow(const XyzProblem &p, int ncut, const float *cutoffs, T *v2, T *
     xyz_correlation<T>(p, cut, v2, v4, rec + n * rsz);
     ++n;
     if (has_nan(p.nw, dv2.data()) || has_nan(nv4, dv4.data())) return -n;
-    // XYZFrgCore::finalizeStep (src/XYZ/XYZFrgCore.cpp:127-150)
+    // XYZFrgCore::finalizeStep (src/XYZ/XYZFrgCore.cpp:128-162)
     const T step = T(cutoffs[k + 1]) - cut;
     for (int i = 0; i < p.nw; ++i) v2[i] += step * dv2[i];
     for (long i = 0; i < nv4; ++i) v4[i] += step * dv4[i];

@@ -115,7 +115,7 @@ __global__ void k_xyz_weights(XyzDev F, int corr_only) {
   channel_points(x, F.mesh, F.nw, F.cut, F.v2, F.dv2, F.Wwp, F.Wc, F.Wn, F.emax, F.flags + 1);
 }
 
-// getValue(i1, i2, s, t, u, c, FrequencyChannel::None) (XYZVertexTwoParticle.hpp:157-285)
+// getValue(i1, i2, s, t, u, c, FrequencyChannel::None) (XYZVertexTwoParticle.hpp:188-310)
 // for pairs whose transformation keeps the component (the density, or the
 // pair (0, 0)); rid_fwd / rid_inv = symmetryTransform(i1, i2) / (i2, i1)
 __device__ double xyz_value_none(const double *G, long long np, const double *m, int nw, int L, int rid_fwd,
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(64) void k_xyz_dsigma(XyzDev F) {
 }
 
 // frequency arguments of access buffer k in channel ch (0 S, 1 T, 2 U) and the
-// channel of exactness (src/XYZ/XYZFrgCore.cpp:219-227, 267-275, 311-318, 420-427).
+// channel of exactness (src/XYZ/XYZFrgCore.cpp:226-230, 281-287, 325-331, 423-427).
 // T: k 0..3 the bundles, 4..7 the site-local vertices Cb, ICa, Cb2, ICa2.
 __device__ __forceinline__ void xyz_args(int ch, int k, double s, double t, double u, double w1, double w1p, double w2,
                                          double w2p, double wp, double &a, double &b, double &c, int &chan) {
@@ -217,7 +217,7 @@ __device__ __forceinline__ int xyz_support(int so, int to, int uo, int nw, int L
 
 // chalice / inverse chalice factor of spin output c: the two other spin
 // components of the local vertex enter with +, the component itself and the
-// density with - (XYZFrgCore.cpp:336-403)
+// density with - (XYZFrgCore.cpp:333-416)
 __device__ __forceinline__ double chal(const double *l, int c) {
   return (l[0] + l[1] + l[2]) - 2.0 * l[c] - l[kD];
 }
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
   for (int i = tid; i < nw; i += kXyzThreads) s_mesh[i] = F.mesh[i];
   // element offsets (plane of the component, site) for the plain and the
   // exchanged bundle: spin components through the site's permutation
-  // (XYZVertexTwoParticle.hpp:376-397)
+  // (XYZVertexTwoParticle.hpp:385-407)
   for (int q = tid; q < 4 * L; q += kXyzThreads) {
     const int c = q / L, j = q - (q / L) * L;
     const int pf = c < 3 ? F.site_perm[3 * j + c] : kD, pi = c < 3 ? F.inv_perm[3 * j + c] : kD;
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
           for (int q = 0; q < 4; ++q) { s_off[d * 4 + q] = off[q]; s_w[d * 4 + q] = w[q]; }
           s_fl[d] = fl;
         } else {
-          // getValueLocal (XYZVertexTwoParticle.hpp:340-362): site 0
+          // getValueLocal (XYZVertexTwoParticle.hpp:355-376): site 0
 #pragma unroll
           for (int cc = 0; cc < 4; ++cc) {
             double v = 0.0;
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
         for (int k = 0; k < 4; ++k) { a[k] = cw * Xb[k * L + cj]; b[k] = Yb[k * L + cj]; }
         if (ch != 1) {
           // particle-particle (S, signs +) and crossed particle-hole (U, all -)
-          // ladders (XYZFrgCore.cpp:229-264, 320-355)
+          // ladders (XYZFrgCore.cpp:223-275, 419-470)
           const double sx = ch == 0 ? 1.0 : -1.0;
           acc[kX] = fma(sx, fma(a[kD], b[kX], a[kX] * b[kD]), acc[kX]);
           acc[kX] = fma(-1.0, fma(a[kZ], b[kY], a[kY] * b[kZ]), acc[kX]);
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
           acc[kZ] = fma(-1.0, fma(a[kX], b[kY], a[kY] * b[kX]), acc[kZ]);
           acc[kD] = fma(sx, fma(a[kD], b[kD], fma(a[kX], b[kX], fma(a[kY], b[kY], a[kZ] * b[kZ]))), acc[kD]);
         } else {
-          // lattice bubble of the first pair (XYZFrgCore.cpp:277-303): entry i
+          // lattice bubble of the first pair (XYZFrgCore.cpp:295-322): entry i
           // of getOverlap(rid) pairs site rid1 of bundle 0 and rid2 of bundle 1,
           // each spin component through the entry's transformation; the two
           // pair threads of a site take halves of its entries
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
 #pragma unroll
           for (int k = 0; k < 4; ++k) acc[k] = fma(c4, r[k], acc[k]);
           // chalice with Cb (pair 0) / Cb2 (pair 1) and inverse chalice with
-          // ICa / ICa2 (XYZFrgCore.cpp:305-403)
+          // ICa / ICa2 (XYZFrgCore.cpp:324-416)
           const double *lc = s_loc + (ce * 4 + 2 * cpair) * 4;
           const double *li = s_loc + (ce * 4 + 2 * cpair + 1) * 4;
           double Lc[4], Li[4];

@@ -3,8 +3,10 @@
 
 ``write_correlation`` appends one measurement of one observable group, exactly
 as ``SU2MeasurementCorrelation::_writeOutfileCorrelation`` /
+``XYZMeasurementCorrelation::_writeOutfileCorrelation`` /
 ``TRIMeasurementCorrelation::_writeOutfileCorrelation`` do
 (``src/SU2/SU2MeasurementCorrelation.cpp:179-355``,
+``src/XYZ/XYZMeasurementCorrelation.cpp:214-390``,
 ``src/TRI/TRIMeasurementCorrelation.cpp:318-493``): meta group with the lattice
 vectors, basis and site positions, then ``data/measurement_<k>`` with the
 float ``cutoff`` attribute and the (basis, sites in range) float data; a
@@ -127,6 +129,12 @@ def write_tri_records(path: str, lattice, records: Iterable, groups: Optional[It
             ok = write_correlation(path, g, r.cutoff, lattice, r.data[g]) and ok
         n += int(ok)
     return n
+
+
+def write_xyz_records(path: str, lattice, records: Iterable) -> int:
+    """The four XYZCor* groups (``src/XYZ/XYZMeasurementCorrelation.cpp:85-96``)
+    from records keyed by group name."""
+    return write_tri_records(path, lattice, records, ("XYZCorXX", "XYZCorYY", "XYZCorZZ", "XYZCorDD"))
 
 
 SP_OBS_DUPLICATE_CKPT = -3

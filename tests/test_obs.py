@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from helpers import GOLDEN, ref1_problem, tri_ref3_problem
+from helpers import GOLDEN, ref1_problem, tri_ref3_problem, xyz_ref2_problem
 
 obs = pytest.importorskip("spinparser_amd.obs")
 if not os.path.exists(obs.OBS_PATH):
@@ -79,6 +79,29 @@ def test_tri_golden_flow_roundtrip(tmp_path):
         for k in range(n):
             assert cut[k] == g["%s/data/measurement_%d/cutoff" % (grp, k)][0]
             assert np.abs(data[k] - g["%s/data/measurement_%d/data" % (grp, k)]).max() < 1e-6
+
+
+def test_xyz_golden_flow_bit_exact_file(tmp_path):
+    """XYZ flow of test_reference2 written as XYZCor* groups: every dataset and
+    cutoff attribute equals the golden file bit for bit, and the meta group
+    holds its site positions."""
+    from oracle import XYZ_GROUPS, XyzOracle
+    from spinparser_amd.frgcore import TRICorrelationRecord
+    p = xyz_ref2_problem()
+    rec, n, _ = XyzOracle(p).run_flow(dtype=np.float32, nthreads=8)
+    recs = [TRICorrelationRecord(float(p.cutoffs[k]), {g: rec[k, i] for i, g in enumerate(XYZ_GROUPS)})
+            for k in range(n)]
+    path = str(tmp_path / "ref2.obs")
+    assert obs.write_xyz_records(path, p.lattice, recs) == n
+    g = np.load(GOLDEN + "/reference2_xyz.npz")
+    for grp in XYZ_GROUPS:
+        cut, data = obs.read_correlations(path, grp, (p.B, p.Rb))
+        assert len(cut) == n
+        for k in range(n):
+            assert cut[k] == g["%s/data/measurement_%d/cutoff" % (grp, k)][0]
+            assert np.array_equal(data[k], g["%s/data/measurement_%d/data" % (grp, k)])
+    _, _, sites = obs.lattice_geometry(p.lattice)
+    assert np.allclose(sites, g["XYZCorXX/meta/sites"], atol=1e-6)
 
 
 def test_writer_exports_header_symbols():
