@@ -34,9 +34,13 @@ dGamma/dLambda evaluations per second (``value``, FP64 form).  Beside it:
 * ``tri``: the TRI core on kagome J = 1, D = 0.2 (range 5, 48 frequencies,
   cutoff 50 -> 0.01 x 0.98): evaluations at five cutoffs over the schedule and
   the full-schedule rate integrated by algorithmic FLOP, with its own CPU
-  baseline (float TRI oracle on a bounded item sample).
+  baseline (float TRI oracle on a bounded item sample);
+* ``xyz``: the same for the XYZ core on the honeycomb Heisenberg-Kitaev model
+  J = 0.1, K = -1 (range 8: L = 23, 1,525 overlap terms; 64 frequencies in
+  [0.005, 50], cutoff 50 -> 0.01 x 0.98).
 
-Mode "tri-flow": the whole TRI flow (421 evaluations) measured end to end.
+Modes "tri-flow" / "xyz-flow": the whole TRI / XYZ flow (421 evaluations)
+measured end to end, with roofline and CPU baseline.
 
 Mode "sweep": the served J2/J1 phase-diagram sweep (64 SU(2) cubic flows, J2 ~
 U[0,1], range ~ U{3..7}, N_w ~ U{32,48,64}, Poisson arrivals, mixed precision,
@@ -56,8 +60,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from spinparser_amd.capi import (KERNEL_DGAMMA, SP_PRECISION_FP64, SP_PRECISION_MIXED,  # noqa: E402
-                                 FlowContext, TriFlowContext, footprint_bytes)
-from spinparser_amd.problem import su2_problem, tri_problem  # noqa: E402
+                                 FlowContext, TriFlowContext, XyzFlowContext, footprint_bytes)
+from spinparser_amd.problem import su2_problem, tri_problem, xyz_problem  # noqa: E402
 from spinparser_amd.task import make_task_xml, parse_task  # noqa: E402
 from spinparser_amd.workload import compulsory_bytes, evaluation_flops  # noqa: E402
 
@@ -79,6 +83,20 @@ WORKLOADS = {
 # cutoff 50 -> 0.01 with step 0.98 (422 cutoffs)
 TRI_WORKLOAD = dict(lattice="kagome", lattice_range=5, model="kagome-DM", couplings={"j": 1.0, "d": 0.2},
                     freq_min=0.005, freq_max=50.0, freq_count=48, cutoff_max=50.0, cutoff_min=0.01, cutoff_step=0.98)
+# XYZ core: honeycomb Heisenberg-Kitaev J = 0.1, K = -1, range 8 (L = 23, 1,525
+# overlap terms, all six spin permutations among the overlap transformations),
+# 64 log-spaced frequencies in [0.005, 50], cutoff 50 -> 0.01 with step 0.98
+XYZ_WORKLOAD = dict(lattice="honeycomb", lattice_range=8, model="honeycomb-kitaev", couplings={"j": 0.1, "k": -1.0},
+                    freq_min=0.005, freq_max=50.0, freq_count=64, cutoff_max=50.0, cutoff_min=0.01, cutoff_step=0.98)
+
+# the cores measured beside the SU(2) flow: workload, problem builder, context,
+# oracle class name and dGamma kernel name
+CORES = {
+    "tri": dict(label="TRI", workload="kagome-dm-r5-nw48", spec=TRI_WORKLOAD, symmetry="TRI", build=tri_problem,
+                context=TriFlowContext, oracle="TriOracle", kernel="k_tri_dgamma"),
+    "xyz": dict(label="XYZ", workload="honeycomb-kitaev-r8-nw64", spec=XYZ_WORKLOAD, symmetry="XYZ",
+                build=xyz_problem, context=XyzFlowContext, oracle="XyzOracle", kernel="k_xyz_dgamma"),
+}
 
 # HBM bytes per k_dgamma launch over the full flow, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over the 655 dispatches
@@ -100,9 +118,10 @@ def build_problem(name):
     return su2_problem(parse_task(xml, is_text=True))
 
 
-def build_tri_problem():
-    xml = make_task_xml(symmetry="TRI", **TRI_WORKLOAD)
-    return tri_problem(parse_task(xml, is_text=True))
+def build_core_problem(core):
+    c = CORES[core]
+    xml = make_task_xml(symmetry=c["symmetry"], **c["spec"])
+    return c["build"](parse_task(xml, is_text=True))
 
 
 def max_rel(a, b):
@@ -248,17 +267,20 @@ def traffic_entry(workload, dtype):
     return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
 
 
-def tri_leg(samples=5, reps=2):
-    """TRI core on the kagome-DM workload: dGamma evaluations at `samples`
-    cutoffs spread over the schedule (state = initial condition placed at the
-    cutoff; the work of an evaluation depends on the cutoff only), and the
+def core_leg(core, samples=5, reps=2):
+    """TRI or XYZ core on its workload: dGamma evaluations at `samples` cutoffs
+    spread over the schedule (state = initial condition placed at the cutoff;
+    the work of an evaluation depends on the cutoff only), and the
     full-schedule rate obtained by integrating the measured time per
     algorithmic FLOP over the algorithmic work of every evaluation."""
-    p = build_tri_problem()
+    c = CORES[core]
+    p = build_core_problem(core)
     ncut = len(p.cutoffs)
     ks = [int(round(x)) for x in np.linspace(0, ncut - 2, samples)]
     v2, v4 = p.initial_state(np.float64)
-    ctx = TriFlowContext(p)
+    ctx = c["context"](p)
+    kkey = c["kernel"] + "_ms"
+    tkey = "tflops_" + c["kernel"]
     pts = []
     for k in ks:
         ctx.set_state(float(p.cutoffs[k]), v2, v4)
@@ -276,14 +298,14 @@ def tri_leg(samples=5, reps=2):
         f = evaluation_flops(p, p.cutoffs[k])
         kern = (ms1 - ms0) * 1e-3 / max(1, n1 - n0)
         pts.append({"cutoff": float(p.cutoffs[k]), "index": k, "gflop_per_eval": f / 1e9, "evals_per_s": 1.0 / wall,
-                    "k_tri_dgamma_ms": kern * 1e3, "tflops_k_tri_dgamma": f / kern / 1e12})
+                    kkey: kern * 1e3, tkey: f / kern / 1e12})
     ctx.close()
     # seconds per FLOP, piecewise linear in the cutoff index between samples
     idx = np.array([q["index"] for q in pts], dtype=float)
     spf = np.array([1.0 / (q["evals_per_s"] * q["gflop_per_eval"] * 1e9) for q in pts])
     flops = np.array([evaluation_flops(p, p.cutoffs[k]) for k in range(ncut - 1)])
     est = float(np.sum(flops * np.interp(np.arange(ncut - 1), idx, spf)))
-    return {"workload": "kagome-dm-r5-nw48", "L": p.L, "nw": p.nw, "overlap_terms": p.overlap_total,
+    return {"workload": c["workload"], "L": p.L, "nw": p.nw, "overlap_terms": p.overlap_total,
             "n_freq_triples": p.n_freq, "cutoffs": ncut, "samples": pts,
             "full_flow_algorithmic_tflop": float(flops.sum()) / 1e12,
             "est_full_flow_s": est, "est_evals_per_s": (ncut - 1) / est,
@@ -291,14 +313,16 @@ def tri_leg(samples=5, reps=2):
                       % (reps, samples)}
 
 
-def tri_cpu_baseline(target_s=10.0):
-    """float TRI oracle (reference arithmetic, OpenMP guided over items) on a
-    bounded item sample at five cutoffs, extrapolated by algorithmic FLOP."""
+def core_cpu_baseline(core, target_s=10.0):
+    """float CPU oracle of the TRI / XYZ core (reference arithmetic, OpenMP
+    guided over items) on a bounded item sample at five cutoffs, extrapolated
+    by algorithmic FLOP."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import TriOracle
+    import oracle
+    c = CORES[core]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    p = build_tri_problem()
-    o = TriOracle(p)
+    p = build_core_problem(core)
+    o = getattr(oracle, c["oracle"])(p)
     v2, v4 = p.initial_state(np.float32)
     dv2 = np.zeros(p.nw, dtype=np.float32)
     ncut = len(p.cutoffs)
@@ -319,18 +343,19 @@ def tri_cpu_baseline(target_s=10.0):
     flow_flops = sum(evaluation_flops(p, p.cutoffs[k]) for k in range(ncut - 1))
     flow_s = t_tot / f_tot * flow_flops
     return {"value": (ncut - 1) / flow_s, "unit": "dGamma/dLambda evals/s", "cores": threads, "kind": "port",
-            "sample": "float32 TRI CPU oracle on %d items at each of 5 cutoffs (%.1f s measured, %.3g GFLOP/s); "
-                      "full flow %.0f s extrapolated by algorithmic FLOP" % (n_items, t_tot, f_tot / t_tot / 1e9,
-                                                                          flow_s)}
+            "sample": "float32 %s CPU oracle on %d items at each of 5 cutoffs (%.1f s measured, %.3g GFLOP/s); "
+                      "full flow %.0f s extrapolated by algorithmic FLOP" % (c["label"], n_items, t_tot,
+                                                                          f_tot / t_tot / 1e9, flow_s)}
 
 
-def mode_tri_flow(args):
-    """The whole TRI flow on the GPU, one step = computeStep + correlation +
-    isDiverged + finalizeStep."""
-    p = build_tri_problem()
+def mode_core_flow(args, core):
+    """The whole TRI / XYZ flow on the GPU, one step = computeStep + correlation
+    + isDiverged + finalizeStep."""
+    c = CORES[core]
+    p = build_core_problem(core)
     nsteps_flow = len(p.cutoffs) - 1
     steps = min(args.steps, nsteps_flow)
-    ctx = TriFlowContext(p)
+    ctx = c["context"](p)
     v2, v4 = p.initial_state(np.float64)
     ctx.set_state(float(p.cutoffs[0]), v2, v4)
     for w in range(args.warmup):
@@ -346,30 +371,32 @@ def mode_tri_flow(args):
         ctx.compute_step()
         ctx.measure_correlation()
         if ctx.is_diverged():
-            raise RuntimeError("TRI flow diverged at cutoff index %d" % k)
+            raise RuntimeError("%s flow diverged at cutoff index %d" % (c["label"], k))
         ctx.finalize_step(float(p.cutoffs[k + 1]))
         flops += evaluation_flops(p, p.cutoffs[k])
         if (k + 1) % 20 == 0:
-            print("# tri step %d / %d  %.1f s" % (k + 1, steps, time.perf_counter() - t0), file=sys.stderr, flush=True)
+            print("# %s step %d / %d  %.1f s" % (core, k + 1, steps, time.perf_counter() - t0), file=sys.stderr,
+                  flush=True)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     dg_ms, dg_n = ctx.profile_get(KERNEL_DGAMMA)
     ctx.close()
     achieved = flops / (dg_ms * 1e-3) / 1e12
-    out = {"metric": "TRI dGamma/dLambda evals/s (full-schedule average, FP64 flow)", "value": steps / elapsed,
+    out = {"metric": "%s dGamma/dLambda evals/s (full-schedule average, FP64 flow)" % c["label"],
+           "value": steps / elapsed,
            "unit": "evals/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": elapsed * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic (task-file generated flow, initial vertex = model couplings)",
-           "config": {"workload": "kagome-dm-r5-nw48", "L": p.L, "nw": p.nw, "overlap_terms": p.overlap_total,
+           "config": {"workload": c["workload"], "L": p.L, "nw": p.nw, "overlap_terms": p.overlap_total,
                       "cutoffs": len(p.cutoffs), "step": "computeStep + correlation + isDiverged + finalizeStep"},
            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_entry("kagome-dm-r5-nw48", "f64"),
-                        "kernel": "k_tri_dgamma",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_entry(c["workload"], "f64"),
+                        "kernel": c["kernel"],
                         "launches": dg_n, "avg_launch_ms": dg_ms / max(dg_n, 1),
                         "algorithmic_tflop_total": flops / 1e12},
            "cpu_baseline": None}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = tri_cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = core_cpu_baseline(core, args.cpu_seconds)
     print(json.dumps(out))
 
 
@@ -427,12 +454,13 @@ def mode_flow(args):
         mid = float(prob.cutoffs[(nsteps_flow - 1) // 2])
         mixed["max_rel_err_derivative_initial_state_mid_cutoff"] = derivative_errors(prob, (mid, *init))
         out["mixed"] = mixed
-    if not args.no_tri:
-        out["tri"] = tri_leg()
+    legs = [c for c, off in (("tri", args.no_tri), ("xyz", args.no_xyz)) if not off]
+    for c in legs:
+        out[c] = core_leg(c)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
-        if not args.no_tri:
-            out["tri"]["cpu_baseline"] = tri_cpu_baseline(min(10.0, args.cpu_seconds))
+        for c in legs:
+            out[c]["cpu_baseline"] = core_cpu_baseline(c, min(10.0, args.cpu_seconds))
     print(json.dumps(out))
 
 
@@ -512,12 +540,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=655)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="flow", choices=("flow", "sweep", "tri-flow"))
+    ap.add_argument("--mode", default="flow", choices=("flow", "sweep", "tri-flow", "xyz-flow"))
     ap.add_argument("--workload", default="cubic-j1j2-r6-nw64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-fixed", action="store_true")
     ap.add_argument("--no-tri", action="store_true")
+    ap.add_argument("--no-xyz", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--flows", type=int, default=64)
     ap.add_argument("--seed", type=int, default=2024)
@@ -533,7 +562,9 @@ def main():
     if args.mode == "flow":
         mode_flow(args)
     elif args.mode == "tri-flow":
-        mode_tri_flow(args)
+        mode_core_flow(args, "tri")
+    elif args.mode == "xyz-flow":
+        mode_core_flow(args, "xyz")
     else:
         mode_sweep(args)
 
