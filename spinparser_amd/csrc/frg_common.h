@@ -1,7 +1,9 @@
-// Device helpers shared by the SU(2) and TRI flow kernels: the frequency mesh
-// (bracket searches, interpolation), the self-energy interpolation, the
-// trapezoid evaluation points of ImplicitIntegrator and the per-cutoff tables
-// of evaluation points and propagator weights.
+// Device helpers shared by the SU(2), XYZ and TRI flow kernels: the frequency
+// mesh (bracket searches, interpolation), the self-energy interpolation, the
+// trapezoid evaluation points of ImplicitIntegrator, the per-cutoff tables of
+// evaluation points and propagator weights, and the access-buffer descriptors
+// (support rows, bilinear weights, density signs) of the SU(2) and XYZ vertex
+// layouts.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -180,5 +182,89 @@ __device__ inline void correlation_points(const double *m, int nw, double cut, d
   *Cn = o.cnt < emax ? o.cnt : emax;
   if (o.cnt > emax) atomicOr(overflow, 1);
 }
+
+
+// ---------------------------------------------------------------------------
+// access-buffer descriptors (SU(2) and XYZ layouts: one element per (row, site))
+// ---------------------------------------------------------------------------
+// row index of a support point with the s <-> u exchange of
+// _generateAccessBufferOffset (src/SU2/SU2VertexTwoParticle.hpp:616-632,
+// src/XYZ/XYZVertexTwoParticle.hpp:637-653); neg = density sign flip
+__device__ __forceinline__ int support_row(int so, int to, int uo, int nw, bool &neg) {
+  if (so < uo) { neg = true; return tri(uo, so) * nw + to; }
+  neg = false;
+  return tri(so, uo) * nw + to;
+}
+
+struct Interp {
+  int lo, hi;
+  double b;
+};
+
+__device__ __forceinline__ Interp interp_of(const double *m, int nw, double x) {
+  Interp r;
+  interp(m, nw, x, r.lo, r.hi, r.b);
+  return r;
+}
+
+template <class V2> __device__ __forceinline__ V2 buf_load(__amdgpu_buffer_rsrc_t r, int byte_off);
+
+template <> __device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
+  double2 d;
+  __builtin_memcpy(&d, &v, 16);
+  return d;
+}
+
+template <> __device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+  float2 d;
+  __builtin_memcpy(&d, &v, 8);
+  return d;
+}
+
+// Access-buffer descriptor of generateAccessBuffer(s, t, u, channel)
+// (src/SU2/SU2VertexTwoParticle.hpp:399-490, src/XYZ/XYZVertexTwoParticle.hpp:419-496)
+// from the two interpolations it needs: chan 0 = s exact (I1 on t, I2 on u),
+// 1 = t exact (I1 on s, I2 on u), 2 = u exact (I1 on s, I2 on t).  Support byte
+// offsets are row * rowbytes; the density weight carries the sign of the
+// s <-> u exchange.
+struct Desc4 {
+  int off[4];
+  double ws[4];
+  double wd[4];
+};
+
+__device__ __forceinline__ Desc4 assemble(int chan, int ex, Interp i1, Interp i2, int nw, int rowbytes) {
+  Desc4 d;
+  const int l1[4] = {i1.lo, i1.hi, i1.lo, i1.hi};
+  const int l2[4] = {i2.lo, i2.lo, i2.hi, i2.hi};
+  const double b1 = i1.b, b2 = i2.b;
+  const double w[4] = {(1 - b2) * (1 - b1), (1 - b2) * b1, b2 * (1 - b1), b2 * b1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int so, to, uo;
+    if (chan == 0) { so = ex; to = l1[q]; uo = l2[q]; }
+    else if (chan == 1) { so = l1[q]; to = ex; uo = l2[q]; }
+    else { so = l1[q]; to = l2[q]; uo = ex; }
+    bool neg;
+    const int row = support_row(so, to, uo, nw, neg);
+    d.off[q] = row * rowbytes;
+    d.ws[q] = w[q];
+    d.wd[q] = neg ? -w[q] : w[q];
+  }
+  return d;
+}
+
+// one access buffer of a pair: the signed s and u arguments decide the site
+// exchange (exactly one of them negative); swap says whether the
+// pair's interpolations (I0, I1) feed the descriptor's (slot 1, slot 2) or
+// (slot 2, slot 1)
+struct PairBuf {
+  int k;
+  double a, c;
+  bool swap;
+  int chan;
+};
 
 }  // namespace frg

@@ -101,14 +101,6 @@ constexpr long long kChunk = SPFRG_CHUNK;
 using namespace frg;
 
 
-// row index of a support point with the s <-> u exchange of
-// SU2VertexTwoParticle::_generateAccessBufferOffset (:616-632); neg = density sign flip
-__device__ __forceinline__ int support_row(int so, int to, int uo, int nw, bool &neg) {
-  if (so < uo) { neg = true; return tri(uo, so) * nw + to; }
-  neg = false;
-  return tri(so, uo) * nw + to;
-}
-
 // vertex value with exact mesh indices, SU2VertexTwoParticle::_directAccessMapFrequencyExchange
 __device__ __forceinline__ double2 direct2(const double2 *G, int L, int nw, int rid, int so, int to, int uo) {
   if (so >= uo) return G[(size_t)(tri(so, uo) * nw + to) * L + rid];
@@ -264,33 +256,6 @@ __host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots
   return l;
 }
 
-struct Interp {
-  int lo, hi;
-  double b;
-};
-
-__device__ __forceinline__ Interp interp_of(const double *m, int nw, double x) {
-  Interp r;
-  interp(m, nw, x, r.lo, r.hi, r.b);
-  return r;
-}
-
-template <class V2> __device__ __forceinline__ V2 buf_load(__amdgpu_buffer_rsrc_t r, int byte_off);
-
-template <> __device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, int byte_off) {
-  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
-  double2 d;
-  __builtin_memcpy(&d, &v, 16);
-  return d;
-}
-
-template <> __device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, int byte_off) {
-  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
-  float2 d;
-  __builtin_memcpy(&d, &v, 8);
-  return d;
-}
-
 template <class V2, class V> __device__ __forceinline__ V2 mk2(V a, V b) {
   V2 r;
   r.x = a;
@@ -309,49 +274,6 @@ __device__ __forceinline__ void ksum_add(KSum &k, float x) {
   k.c += (k.s - (t - z)) + (x - z);
   k.s = t;
 }
-
-// Access-buffer descriptor of SU2VertexTwoParticle::generateAccessBuffer(s, t,
-// u, channel) (:399-490) from the two interpolations it needs: chan 0 = s exact
-// (I1 on t, I2 on u), 1 = t exact (I1 on s, I2 on u), 2 = u exact (I1 on s,
-// I2 on t).  Support byte offsets are row * rowbytes; the density weight
-// carries the sign of the s <-> u exchange (:616-632).
-struct Desc4 {
-  int off[4];
-  double ws[4];
-  double wd[4];
-};
-
-__device__ __forceinline__ Desc4 assemble(int chan, int ex, Interp i1, Interp i2, int nw, int rowbytes) {
-  Desc4 d;
-  const int l1[4] = {i1.lo, i1.hi, i1.lo, i1.hi};
-  const int l2[4] = {i2.lo, i2.lo, i2.hi, i2.hi};
-  const double b1 = i1.b, b2 = i2.b;
-  const double w[4] = {(1 - b2) * (1 - b1), (1 - b2) * b1, b2 * (1 - b1), b2 * b1};
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    int so, to, uo;
-    if (chan == 0) { so = ex; to = l1[q]; uo = l2[q]; }
-    else if (chan == 1) { so = l1[q]; to = ex; uo = l2[q]; }
-    else { so = l1[q]; to = l2[q]; uo = ex; }
-    bool neg;
-    const int row = support_row(so, to, uo, nw, neg);
-    d.off[q] = row * rowbytes;
-    d.ws[q] = w[q];
-    d.wd[q] = neg ? -w[q] : w[q];
-  }
-  return d;
-}
-
-// one access buffer of a pair: the signed s and u arguments decide the site
-// exchange (exactly one of them negative, :405-427); swap says whether the
-// pair's interpolations (I0, I1) feed the descriptor's (slot 1, slot 2) or
-// (slot 2, slot 1)
-struct PairBuf {
-  int k;
-  double a, c;
-  bool swap;
-  int chan;
-};
 
 template <bool MIXED>
 __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOCKS) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,

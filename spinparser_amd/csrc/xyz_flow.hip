@@ -3,7 +3,7 @@
 // MI355X (gfx950), FP64.
 //
 // Reference behaviour followed (paths relative to the reference repository):
-//   src/XYZ/XYZVertexTwoParticle.hpp:63-77,157-640  storage (four planes), access buffers,
+//   src/XYZ/XYZVertexTwoParticle.hpp:63-77,188-653  storage (four planes), access buffers,
 //                                                   density sign of the s <-> u exchange,
 //                                                   site spin permutations
 //   src/XYZ/XYZFrgCore.cpp:128-193                  finalizeStep, dSigma
@@ -12,29 +12,33 @@
 //   src/XYZ/XYZMeasurementCorrelation.cpp:98-212    static susceptibility, 4 outputs
 //
 // Device memory (per flow):
-//   Gamma, dGamma  FP64, the reference's four planes X, Y, Z, D of N_f * L values,
-//                  plane c at c * N_f * L, entry ((s(s+1)/2 + u) * N_w + t) * L + rid
+//   Gamma, dGamma  FP64, components interleaved: the element of (row, site) is
+//                  {X, Y, Z, D} at ((row * L) + rid) * 4, row = (s(s+1)/2 + u) * N_w + t,
+//                  so one support of one site is two 16-byte loads.  The C ABI
+//                  converts from / to the reference's four planes
+//                  (_dataXX ... _dataDD) on upload and download.
 //   Sigma, dSigma  FP64 [N_w]
 //   weight tables  per exact index x: evaluation points and weights at the cutoff
 //
-// dGamma kernel: one workgroup of 256 threads per frequency triple (s, t, u),
-// XCD-chunked like the SU(2) and TRI kernels.  Per channel the evaluation
-// points are processed in chunks of nb points:
-//   A  one thread per (point, access buffer) builds the descriptor (4 supports,
-//      weights, exchange flag, density signs); the T channel's four site-local
-//      vertices (4 components each) are evaluated here directly;
-//   B  one thread per (point, buffer, site) gathers the 4-component bundle into
-//      LDS (4 supports x 4 loads; spin components through the site's
-//      permutation, the density with the exchange sign);
-//   C  one thread per (point, site, pair) applies the channel algebra: S / U 16
-//      products per pair; T the lattice bubble of the first bundle pair (4
-//      products per overlap entry, the entries of a site split between the two
-//      pair threads), chalice and inverse chalice (scalar factors per
-//      component), accumulated with the point's weight.
-// The per-thread sums are reduced through LDS at the end of the item.
+// dGamma kernel (k_xyz_dgamma): one workgroup of 256 threads per frequency
+// triple (s, t, u), XCD-chunked like the SU(2) kernel; threads are (slot, rid)
+// pairs.  Per chunk of evaluation points:
+//   A  one thread per (point, pair of access buffers): the two bracket searches
+//      the pair shares, descriptors (support byte offsets, weights, density
+//      signs, exchange) -> LDS; the T channel's four site-local vertices;
+//   B  per (slot, rid): 4 buffers x 4 supports x 2 loads, the site's spin
+//      permutation, the S / U ladder or the chalice terms; in the T channel
+//      the bundles of the first buffer pair go to LDS;
+//   C  (T channel) the lattice bubble as rank updates
+//      M[a][b] += sum_e c_e A0_e[a] (x) A1_e[b] for every spin-component pair
+//      (a, b) the overlap transformations use (2x2 register tiles over LDS),
+//      contracted with the overlap lists once per item.
+// When the bubble matrices do not fit the LDS of a workgroup, k_xyz_dgamma_direct
+// evaluates the bubble per point from the overlap lists instead.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -49,9 +53,10 @@ namespace {
 using namespace frg;
 
 constexpr int kXyzThreads = 256;
-constexpr int kXyzMaxPoints = 16;  // evaluation points per chunk
+constexpr int kXyzMaxPoints = 16;  // evaluation points per chunk of the direct kernel
 constexpr long long kXyzChunk = 16;
 constexpr int kX = 0, kY = 1, kZ = 2, kD = 3;
+constexpr int kMaxPairs = 10;  // 9 spin-component pairs + density
 
 // ---------------------------------------------------------------------------
 // per-flow descriptor (passed by value)
@@ -61,6 +66,7 @@ struct XyzDev {
   const double *mesh;
   const int *site_rid, *site_perm, *inv_rid, *inv_perm;
   const int *ovl_off, *ovl_rid1, *ovl_rid2, *ovl_perm1, *ovl_perm2;
+  const int4 *ovl_midx;  // per overlap entry: bubble-matrix index of X, Y, Z, D
   const int *range_fwd, *range_inv, *corr_rid, *corr_perm;
   double *Wwp, *Wc;
   int *Wn;
@@ -68,33 +74,14 @@ struct XyzDev {
   int *Cn;
   double *corr_partial, *corr_out;
   int *flags;  // [0] NaN in the flow, [1] evaluation-list overflow
-  int nw, L, R, B, Rb, emax, nb, euler;
+  int nw, L, R, B, Rb, emax, nb, slots, euler;
+  int npairs;
+  int pair[kMaxPairs];  // 4 a + b of each bubble matrix
   long long nf;
-  long long np;  // plane size N_f * L
+  long long np;  // N_f * L
   unsigned g_bytes;
   double cut, step;
 };
-
-// LDS carve-up of k_xyz_dgamma (bytes)
-struct XyzLds {
-  int mesh, cf, ci, off, w, fl, c, loc, A, total;
-};
-__host__ __device__ inline XyzLds xyz_lds(int nw, int L, int nb) {
-  XyzLds l;
-  int o = 0;
-  auto take = [&](int bytes) { int r = o; o += (bytes + 15) & ~15; return r; };
-  l.mesh = take(nw * 8);
-  l.cf = take(4 * L * 4);
-  l.ci = take(4 * L * 4);
-  l.off = take(nb * 8 * 4 * 4);
-  l.w = take(nb * 8 * 4 * 8);
-  l.fl = take(nb * 8 * 4);
-  l.c = take(nb * 8);
-  l.loc = take(nb * 4 * 4 * 8);
-  l.A = take(nb * 4 * 4 * L * 8);  // also holds the final reduction (nb * 2 * 4 * L)
-  l.total = o;
-  return l;
-}
 
 __device__ __forceinline__ double gload(__amdgpu_buffer_rsrc_t r, int byte_off) {
   auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
@@ -118,8 +105,8 @@ __global__ void k_xyz_weights(XyzDev F, int corr_only) {
 // getValue(i1, i2, s, t, u, c, FrequencyChannel::None) (XYZVertexTwoParticle.hpp:188-310)
 // for pairs whose transformation keeps the component (the density, or the
 // pair (0, 0)); rid_fwd / rid_inv = symmetryTransform(i1, i2) / (i2, i1)
-__device__ double xyz_value_none(const double *G, long long np, const double *m, int nw, int L, int rid_fwd,
-                                 int rid_inv, double s, double t, double u, int c) {
+__device__ double xyz_value_none(const double *G, const double *m, int nw, int L, int rid_fwd, int rid_inv, double s,
+                                 double t, double u, int c) {
   bool exch = false;
   if (s < 0 && u < 0) { s = -s; u = -u; }
   else if (s < 0) { s = -s; exch = true; }
@@ -131,13 +118,14 @@ __device__ double xyz_value_none(const double *G, long long np, const double *m,
   interp(m, nw, s, lS, uS, bS);
   interp(m, nw, t, lT, uT, bT);
   interp(m, nw, u, lU, uU, bU);
-  const double *P = G + (size_t)c * np + rid;
+  const double *P = G + (size_t)rid * 4 + c;
+  const size_t rs = (size_t)L * 4;
   auto A = [&](int so, int to, int uo) {
     if (so < uo) {
-      const double v = P[((size_t)tri(uo, so) * nw + to) * L];
+      const double v = P[((size_t)tri(uo, so) * nw + to) * rs];
       return c == kD ? -v : v;
     }
-    return P[((size_t)tri(so, uo) * nw + to) * L];
+    return P[((size_t)tri(so, uo) * nw + to) * rs];
   };
   return (1 - bU) * ((1 - bT) * ((1 - bS) * A(lS, lT, lU) + bS * A(uS, lT, lU)) +
                      bT * ((1 - bS) * A(lS, uT, lU) + bS * A(uS, uT, lU))) +
@@ -156,19 +144,385 @@ __global__ __launch_bounds__(64) void k_xyz_dsigma(XyzDev F) {
   const double w = m[i];
   double sum = 0.0;
   for (int j = lane; j < F.R; j += 64) {
-    sum += xyz_value_none(F.G, F.np, m, nw, L, F.range_fwd[j], F.range_inv[j], w + cut, 0.0, w - cut, kD) -
-           xyz_value_none(F.G, F.np, m, nw, L, F.range_fwd[j], F.range_inv[j], w - cut, 0.0, w + cut, kD);
+    sum += xyz_value_none(F.G, m, nw, L, F.range_fwd[j], F.range_inv[j], w + cut, 0.0, w - cut, kD) -
+           xyz_value_none(F.G, m, nw, L, F.range_fwd[j], F.range_inv[j], w - cut, 0.0, w + cut, kD);
   }
   sum = wave_sum(sum);
   if (lane == 0) {
     double v = -2.0 * sum;
     for (int k = 0; k < 4; ++k)
-      v += xyz_value_none(F.G, F.np, m, nw, L, 0, 0, w + cut, w - cut, 0.0, k) -
-           xyz_value_none(F.G, F.np, m, nw, L, 0, 0, w - cut, w + cut, 0.0, k);
+      v += xyz_value_none(F.G, m, nw, L, 0, 0, w + cut, w - cut, 0.0, k) -
+           xyz_value_none(F.G, m, nw, L, 0, 0, w - cut, w + cut, 0.0, k);
     v /= (kTwoPi * (cut + sigma_at(m, nw, F.v2, cut)));
     F.dv2[i] = v;
     if (isnan(v)) atomicOr(&F.flags[0], 1);
   }
+}
+
+// chalice / inverse chalice factor of spin output c: the two other spin
+// components of the local vertex enter with +, the component itself and the
+// density with - (XYZFrgCore.cpp:333-416)
+__device__ __forceinline__ double chal(const double *l, int c) {
+  return (l[0] + l[1] + l[2]) - 2.0 * l[c] - l[kD];
+}
+
+// S (sx = +1) and U (sx = -1) ladders of one bundle pair (a pre-scaled by the
+// point's weight) (XYZFrgCore.cpp:223-275, 419-470): spin output c gets
+// sx (a_d b_c + a_c b_d) - (a_c1 b_c2 + a_c2 b_c1), the density sx sum_k a_k b_k
+__device__ __forceinline__ void ladder(double (&acc)[4], const double *a, const double *b, double sx) {
+  acc[kX] = fma(sx, fma(a[kD], b[kX], a[kX] * b[kD]), acc[kX]);
+  acc[kX] = fma(-1.0, fma(a[kZ], b[kY], a[kY] * b[kZ]), acc[kX]);
+  acc[kY] = fma(sx, fma(a[kD], b[kY], a[kY] * b[kD]), acc[kY]);
+  acc[kY] = fma(-1.0, fma(a[kZ], b[kX], a[kX] * b[kZ]), acc[kY]);
+  acc[kZ] = fma(sx, fma(a[kD], b[kZ], a[kZ] * b[kD]), acc[kZ]);
+  acc[kZ] = fma(-1.0, fma(a[kX], b[kY], a[kY] * b[kX]), acc[kZ]);
+  acc[kD] = fma(sx, fma(a[kD], b[kD], fma(a[kX], b[kX], fma(a[kY], b[kY], a[kZ] * b[kZ]))), acc[kD]);
+}
+
+// chalice (local lc with bundle a) and inverse chalice (local li with bundle
+// b), both already scaled by the point's weight
+__device__ __forceinline__ void chalices(double (&acc)[4], const double *a, const double *b, const double *lc,
+                                         const double *li) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) acc[k] = fma(a[k], chal(lc, k), fma(b[k], chal(li, k), acc[k]));
+  const double sc = (lc[kD] + lc[kX]) + (lc[kY] + lc[kZ]);
+  const double si = (li[kD] + li[kX]) + (li[kY] + li[kZ]);
+  acc[kD] = fma(-a[kD], sc, fma(-b[kD], si, acc[kD]));
+}
+
+__device__ __forceinline__ double pick3(double u0, double u1, double u2, int i) {
+  return i == 0 ? u0 : (i == 1 ? u1 : u2);
+}
+
+// LDS carve-up of k_xyz_dgamma (bytes), identical on host and device.  Lp = L
+// rounded up to even (2x2 tiles of the bubble matrices).
+struct XyzLds {
+  int mesh, c, off, ws, wd, ex, loc, P, Q, M, rpa, red, total;
+};
+__host__ __device__ inline XyzLds xyz_lds(int nw, int L, int nb, int slots, int npairs) {
+  XyzLds l;
+  const int Lp = L + (L & 1);
+  int o = 0;
+  auto take = [&](int bytes) { int r = o; o += (bytes + 15) & ~15; return r; };
+  l.mesh = take(nw * 8);
+  l.c = take(nb * 8);
+  l.off = take(nb * 4 * 16);
+  l.ws = take(nb * 4 * 4 * 8);
+  l.wd = take(nb * 4 * 4 * 8);
+  l.ex = take(nb * 4 * 4);
+  l.loc = take(nb * 4 * 4 * 8);
+  const int pq = 4 * nb * Lp * 8;
+  const int red = slots * L * 4 * 8;
+  l.P = take(pq > red ? pq : red);  // the final slot reduction reuses P
+  l.red = l.P;
+  l.Q = take(pq);
+  l.M = take(npairs * Lp * Lp * 8);
+  l.rpa = take(4 * L * 8);
+  l.total = o;
+  return l;
+}
+
+__global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const long long total = F.nf;
+  const long long j0 = blockIdx.x >> 3;
+  const long long item = ((j0 / kXyzChunk) * 8 + (blockIdx.x & 7)) * kXyzChunk + (j0 % kXyzChunk);
+  if (item >= total) return;
+
+  const int nw = F.nw, L = F.L, nb = F.nb, slots = F.slots, npairs = F.npairs;
+  const int Lp = L + (L & 1);
+  const XyzLds lay = xyz_lds(nw, L, nb, slots, npairs);
+  double *s_mesh = reinterpret_cast<double *>(lds_raw + lay.mesh);
+  double *s_c = reinterpret_cast<double *>(lds_raw + lay.c);
+  int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off);
+  double *s_ws = reinterpret_cast<double *>(lds_raw + lay.ws);
+  double *s_wd = reinterpret_cast<double *>(lds_raw + lay.wd);
+  int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex);
+  double *s_loc = reinterpret_cast<double *>(lds_raw + lay.loc);
+  double *s_P = reinterpret_cast<double *>(lds_raw + lay.P);
+  double *s_Q = reinterpret_cast<double *>(lds_raw + lay.Q);
+  double *s_M = reinterpret_cast<double *>(lds_raw + lay.M);
+  double *s_rpa = reinterpret_cast<double *>(lds_raw + lay.rpa);
+  double *s_red = reinterpret_cast<double *>(lds_raw + lay.red);
+
+  const __amdgpu_buffer_rsrc_t rG = __builtin_amdgcn_make_buffer_rsrc((void *)F.G, (short)0, (int)F.g_bytes, 0x00020000);
+  const int rowbytes = L * 32;
+  const int tid = threadIdx.x;
+
+  // iterator: item = tri(s,u) * nw + t
+  const int su = (int)(item / nw);
+  const int to = (int)(item - (long long)su * nw);
+  int so = (int)((sqrt(8.0 * su + 1.0) - 1.0) * 0.5);
+  while ((so + 1) * (so + 2) / 2 <= su) ++so;
+  while (so * (so + 1) / 2 > su) --so;
+  const int uo = su - so * (so + 1) / 2;
+
+  for (int i = tid; i < nw; i += kXyzThreads) s_mesh[i] = F.mesh[i];
+  for (int i = tid; i < npairs * Lp * Lp; i += kXyzThreads) s_M[i] = 0.0;
+  for (int i = tid; i < 4 * nb * Lp; i += kXyzThreads) { s_P[i] = 0.0; s_Q[i] = 0.0; }
+  __syncthreads();
+
+  const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
+  const double w1p = 0.5 * (s + t + u);
+  const double w1 = 0.5 * (s - t + u);
+  const double w2p = 0.5 * (s - t - u);
+  const double w2 = 0.5 * (s + t - u);
+
+  const int slot = tid / L;
+  const int rid = tid - slot * L;
+  const bool active = slot < slots;
+  // byte offsets of this thread's site (plain / exchanged) and the packed spin
+  // permutations (2 bits per component) of getSites / getInvertedSites
+  int sr_fwd = 0, sr_inv = 0, pf = 0, pi = 0;
+  if (active) {
+    sr_fwd = F.site_rid[rid] * 32;
+    sr_inv = F.inv_rid[rid] * 32;
+    pf = F.site_perm[3 * rid] | (F.site_perm[3 * rid + 1] << 2) | (F.site_perm[3 * rid + 2] << 4);
+    pi = F.inv_perm[3 * rid] | (F.inv_perm[3 * rid + 1] << 2) | (F.inv_perm[3 * rid + 2] << 4);
+  }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+
+  for (int ch = 0; ch < 3; ++ch) {
+    const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
+    const int n = F.Wn[xo];
+    const double *Wwp = F.Wwp + (size_t)xo * F.emax;
+    const double *Wc = F.Wc + (size_t)xo * F.emax;
+    for (int e0 = 0; e0 < n; e0 += nb) {
+      const int nvalid = n - e0 < nb ? n - e0 : nb;
+      // (A) one thread per (point, pair): the two bracket searches shared by
+      // the 2 (s, u channels) or 4 (t channel) access buffers of the pair
+      // (XYZFrgCore.cpp:226-230, 281-287, 325-331, 423-427)
+      if (tid < 2 * nvalid) {
+        const int e = tid >> 1, pr = tid & 1;
+        const double wp = Wwp[e0 + e];
+        double x0, x1;
+        PairBuf pb[4];
+        int nbufs = 2;
+        if (ch == 0) {
+          if (pr == 0) {
+            x0 = fabs(w1 + wp); x1 = fabs(w2 + wp);
+            pb[0] = {0, s, -w2 - wp, false, 0};
+            pb[1] = {2, s, w1 + wp, true, 0};
+          } else {
+            x0 = fabs(w1p + wp); x1 = fabs(w2p + wp);
+            pb[0] = {1, s, -w2p - wp, false, 0};
+            pb[1] = {3, s, w1p + wp, true, 0};
+          }
+        } else if (ch == 1) {
+          nbufs = 4;
+          if (pr == 0) {
+            x0 = fabs(w1 - wp); x1 = fabs(w1p + wp);
+            pb[0] = {0, w1 - wp, w1p + wp, false, 1};
+            pb[1] = {2, w1p + wp, w1 - wp, true, 1};
+            pb[2] = {5, w1 - wp, -t, false, 2};
+            pb[3] = {7, w1p + wp, -t, true, 2};
+          } else {
+            x0 = fabs(w2p - wp); x1 = fabs(w2 + wp);
+            pb[0] = {1, w2p - wp, -w2 - wp, false, 1};
+            pb[1] = {3, w2 + wp, wp - w2p, true, 1};
+            pb[2] = {4, w2p - wp, t, false, 2};
+            pb[3] = {6, w2 + wp, t, true, 2};
+          }
+        } else {
+          if (pr == 0) {
+            x0 = fabs(w1 + wp); x1 = fabs(w2p - wp);
+            pb[0] = {0, w1 + wp, u, false, 2};
+            pb[1] = {2, w2p - wp, u, true, 2};
+          } else {
+            x0 = fabs(w1p + wp); x1 = fabs(w2 - wp);
+            pb[0] = {1, w1p + wp, u, false, 2};
+            pb[1] = {3, w2 - wp, u, true, 2};
+          }
+        }
+        const Interp i0 = interp_of(s_mesh, nw, x0);
+        const Interp i1 = interp_of(s_mesh, nw, x1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (q >= nbufs) break;
+          const PairBuf &b = pb[q];
+          const Desc4 d = assemble(b.chan, xo, b.swap ? i1 : i0, b.swap ? i0 : i1, nw, rowbytes);
+          if (b.k < 4) {
+            const int sk = e * 4 + b.k;
+            s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) { s_ws[sk * 4 + r] = d.ws[r]; s_wd[sk * 4 + r] = d.wd[r]; }
+            s_ex[sk] = ((b.a < 0) != (b.c < 0)) ? 1 : 0;
+          } else {
+            // getValueLocal (XYZVertexTwoParticle.hpp:355-376): site 0, no exchange
+            double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const double2 v01 = buf_load<double2>(rG, d.off[r]);
+              const double2 v23 = buf_load<double2>(rG, d.off[r] + 16);
+              l0 = fma(d.ws[r], v01.x, l0);
+              l1 = fma(d.ws[r], v01.y, l1);
+              l2 = fma(d.ws[r], v23.x, l2);
+              l3 = fma(d.wd[r], v23.y, l3);
+            }
+            double *lo = s_loc + (e * 4 + (b.k - 4)) * 4;
+            lo[0] = l0; lo[1] = l1; lo[2] = l2; lo[3] = l3;
+          }
+        }
+        if (pr == 0) s_c[e] = Wc[e0 + e];
+      }
+      __syncthreads();
+      // (B) gathers and channel algebra per (slot, rid)
+      if (active) {
+        for (int e = slot; e < nvalid; e += slots) {
+          double A[4][4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int sk = e * 4 + k;
+            const bool ex = s_ex[sk] != 0;
+            const int sr = ex ? sr_inv : sr_fwd;
+            const int pm = ex ? pi : pf;
+            const int4 o = s_off[sk];
+            const int oo[4] = {o.x, o.y, o.z, o.w};
+            double u0 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const double2 v01 = buf_load<double2>(rG, oo[r] + sr);
+              const double2 v23 = buf_load<double2>(rG, oo[r] + sr + 16);
+              const double ws = s_ws[sk * 4 + r], wd = s_wd[sk * 4 + r];
+              u0 = fma(ws, v01.x, u0);
+              u1 = fma(ws, v01.y, u1);
+              u2 = fma(ws, v23.x, u2);
+              u3 = fma(wd, v23.y, u3);
+            }
+            // bundle component c of the site is the stored component perm[c]
+            // (getValueSuperbundle, XYZVertexTwoParticle.hpp:385-407)
+            A[k][kX] = pick3(u0, u1, u2, pm & 3);
+            A[k][kY] = pick3(u0, u1, u2, (pm >> 2) & 3);
+            A[k][kZ] = pick3(u0, u1, u2, (pm >> 4) & 3);
+            A[k][kD] = u3;
+          }
+          const double c = s_c[e];
+          double a0[4], b1[4], a2[4], b3[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            a0[q] = c * A[0][q]; a2[q] = c * A[2][q];
+            b1[q] = A[1][q]; b3[q] = A[3][q];
+          }
+          if (ch != 1) {
+            const double sx = ch == 0 ? 1.0 : -1.0;
+            ladder(acc, a0, b1, sx);
+            ladder(acc, a2, b3, sx);
+          } else {
+            // chalice with Cb / Cb2, inverse chalice with ICa / ICa2 (locals 4..7)
+            const double *lc = s_loc + e * 16;
+            double cb1[4], cb3[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { cb1[q] = c * b1[q]; cb3[q] = c * b3[q]; }
+            chalices(acc, a0, cb1, lc, lc + 4);
+            chalices(acc, a2, cb3, lc + 8, lc + 12);
+            // lattice-bubble factors of the first pair: P = c A0, Q = A1
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              s_P[(q * nb + e) * Lp + rid] = a0[q];
+              s_Q[(q * nb + e) * Lp + rid] = b1[q];
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (ch == 1) {
+        // (C) M[k] += sum_e P[a_k][e] (x) Q[b_k][e], 2x2 register tiles
+        const int hp = Lp >> 1;
+        const int tiles = npairs * hp * hp;
+        for (int j = tid; j < tiles; j += kXyzThreads) {
+          const int k = j / (hp * hp);
+          const int r = j - k * hp * hp;
+          const int a = 2 * (r / hp), b = 2 * (r - (r / hp) * hp);
+          const int pa = F.pair[k] >> 2, pbq = F.pair[k] & 3;
+          const double *P = s_P + pa * nb * Lp + a;
+          const double *Q = s_Q + pbq * nb * Lp + b;
+          double *M0 = s_M + k * Lp * Lp + a * Lp + b;
+          double *M1 = M0 + Lp;
+          double2 c0 = *reinterpret_cast<double2 *>(M0), c1 = *reinterpret_cast<double2 *>(M1);
+          for (int e = 0; e < nvalid; ++e) {
+            const double2 p = *reinterpret_cast<const double2 *>(P + e * Lp);
+            const double2 q = *reinterpret_cast<const double2 *>(Q + e * Lp);
+            c0.x = fma(p.x, q.x, c0.x);
+            c0.y = fma(p.x, q.y, c0.y);
+            c1.x = fma(p.y, q.x, c1.x);
+            c1.y = fma(p.y, q.y, c1.y);
+          }
+          *reinterpret_cast<double2 *>(M0) = c0;
+          *reinterpret_cast<double2 *>(M1) = c1;
+        }
+        __syncthreads();
+      }
+    }
+    if (ch == 1) {
+      // lattice bubble (XYZFrgCore.cpp:295-322): RPA_c(rid) = sum over
+      // getOverlap(rid) of M[pair of the entry's transforms of c][rid1][rid2], x 4
+      const int wave = tid >> 6, lane = tid & 63;
+      for (int r = wave; r < L; r += kXyzThreads / 64) {
+        double sx = 0.0, sy = 0.0, sz = 0.0, sd = 0.0;
+        const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
+        for (int i = b + lane; i < e; i += 64) {
+          const int4 m = F.ovl_midx[i];
+          sx += s_M[m.x];
+          sy += s_M[m.y];
+          sz += s_M[m.z];
+          sd += s_M[m.w];
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        sz = wave_sum(sz);
+        sd = wave_sum(sd);
+        if (lane == 0) {
+          s_rpa[4 * r + 0] = 4.0 * sx;
+          s_rpa[4 * r + 1] = 4.0 * sy;
+          s_rpa[4 * r + 2] = 4.0 * sz;
+          s_rpa[4 * r + 3] = 4.0 * sd;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // reduce the slots; prefactor 1/(2 pi) (XYZFrgCore.cpp:564-570)
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s_red[(slot * L + rid) * 4 + k] = acc[k];
+  }
+  __syncthreads();
+  for (int q = tid; q < 4 * L; q += kXyzThreads) {
+    const int r = q >> 2, k = q & 3;
+    double v = s_rpa[q];
+    for (int sl = 0; sl < slots; ++sl) v += s_red[(sl * L + r) * 4 + k];
+    v *= (1.0 / kTwoPi);
+    F.dG[(size_t)item * 4 * L + q] = v;
+    if (isnan(v)) atomicOr(&F.flags[0], 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// direct-bubble dGamma kernel, for lattices whose bubble matrices exceed the
+// LDS of a workgroup.  One thread per (point, access buffer) for the
+// descriptors, one per (point, buffer, site) for the gathers into LDS, one per
+// (point, site, pair) for the algebra; the bubble is summed per point from the
+// overlap lists, the entries of a site split between its two pair threads.
+// ---------------------------------------------------------------------------
+struct XyzLdsDirect {
+  int mesh, cf, ci, off, w, fl, c, loc, A, total;
+};
+__host__ __device__ inline XyzLdsDirect xyz_lds_direct(int nw, int L, int nb) {
+  XyzLdsDirect l;
+  int o = 0;
+  auto take = [&](int bytes) { int r = o; o += (bytes + 15) & ~15; return r; };
+  l.mesh = take(nw * 8);
+  l.cf = take(4 * L * 4);
+  l.ci = take(4 * L * 4);
+  l.off = take(nb * 8 * 4 * 4);
+  l.w = take(nb * 8 * 4 * 8);
+  l.fl = take(nb * 8 * 4);
+  l.c = take(nb * 8);
+  l.loc = take(nb * 4 * 4 * 8);
+  l.A = take(nb * 4 * 4 * L * 8);  // also holds the final reduction (nb * 2 * 4 * L)
+  l.total = o;
+  return l;
 }
 
 // frequency arguments of access buffer k in channel ch (0 S, 1 T, 2 U) and the
@@ -208,21 +562,7 @@ __device__ __forceinline__ void xyz_args(int ch, int k, double s, double t, doub
   }
 }
 
-// support row offset (elements within a plane) with the s <-> u exchange flag
-__device__ __forceinline__ int xyz_support(int so, int to, int uo, int nw, int L, bool &fx) {
-  fx = so < uo;
-  const int row = fx ? tri(uo, so) * nw + to : tri(so, uo) * nw + to;
-  return row * L;
-}
-
-// chalice / inverse chalice factor of spin output c: the two other spin
-// components of the local vertex enter with +, the component itself and the
-// density with - (XYZFrgCore.cpp:333-416)
-__device__ __forceinline__ double chal(const double *l, int c) {
-  return (l[0] + l[1] + l[2]) - 2.0 * l[c] - l[kD];
-}
-
-__global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
+__global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma_direct(XyzDev F) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const long long total = F.nf;
   const long long j0 = blockIdx.x >> 3;
@@ -230,8 +570,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
   if (item >= total) return;
 
   const int nw = F.nw, L = F.L, nb = F.nb;
-  const int np = (int)F.np;
-  const XyzLds lay = xyz_lds(nw, L, nb);
+  const XyzLdsDirect lay = xyz_lds_direct(nw, L, nb);
   double *s_mesh = reinterpret_cast<double *>(lds_raw + lay.mesh);
   int *s_cf = reinterpret_cast<int *>(lds_raw + lay.cf);
   int *s_ci = reinterpret_cast<int *>(lds_raw + lay.ci);
@@ -245,7 +584,6 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
   const __amdgpu_buffer_rsrc_t rG = __builtin_amdgcn_make_buffer_rsrc((void *)F.G, (short)0, (int)F.g_bytes, 0x00020000);
   const int tid = threadIdx.x;
 
-  // iterator: item = tri(s,u) * nw + t
   const int su = (int)(item / nw);
   const int to = (int)(item - (long long)su * nw);
   int so = (int)((sqrt(8.0 * su + 1.0) - 1.0) * 0.5);
@@ -254,14 +592,13 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
   const int uo = su - so * (so + 1) / 2;
 
   for (int i = tid; i < nw; i += kXyzThreads) s_mesh[i] = F.mesh[i];
-  // element offsets (plane of the component, site) for the plain and the
-  // exchanged bundle: spin components through the site's permutation
-  // (XYZVertexTwoParticle.hpp:385-407)
+  // element offsets (site, component) for the plain and the exchanged bundle:
+  // spin components through the site's permutation
   for (int q = tid; q < 4 * L; q += kXyzThreads) {
     const int c = q / L, j = q - (q / L) * L;
     const int pf = c < 3 ? F.site_perm[3 * j + c] : kD, pi = c < 3 ? F.inv_perm[3 * j + c] : kD;
-    s_cf[q] = pf * np + F.site_rid[j];
-    s_ci[q] = pi * np + F.inv_rid[j];
+    s_cf[q] = F.site_rid[j] * 4 + pf;
+    s_ci[q] = F.inv_rid[j] * 4 + pi;
   }
   __syncthreads();
 
@@ -271,13 +608,13 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
   const double w2p = 0.5 * (s - t - u);
   const double w2 = 0.5 * (s + t - u);
 
-  // phase C role: (point, site, pair)
   const int ce = tid / (2 * L);
   const int crem = tid - ce * 2 * L;
   const int cpair = crem / L;
   const int cj = crem - cpair * L;
   const bool cactive = ce < nb;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int row4 = 4 * L;  // elements per row
 
   for (int ch = 0; ch < 3; ++ch) {
     const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
@@ -287,7 +624,6 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
     const int nk = ch == 1 ? 8 : 4;
     for (int e0 = 0; e0 < n; e0 += nb) {
       const int nvalid = n - e0 < nb ? n - e0 : nb;
-      // (A) descriptors; site-local vertices of the T channel
       if (tid < nvalid * nk) {
         const int e = tid / nk, k = tid - (tid / nk) * nk;
         const double wp = Wwp[e0 + e];
@@ -309,9 +645,11 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
           const int l1 = (q & 1) ? hi1 : lo1, l2 = (q & 2) ? hi2 : lo2;
           w[q] = ((q & 2) ? b2 : (1 - b2)) * ((q & 1) ? b1 : (1 - b1));
           bool fx;
-          if (chan == 0) off[q] = xyz_support(xo, l1, l2, nw, L, fx);
-          else if (chan == 1) off[q] = xyz_support(l1, xo, l2, nw, L, fx);
-          else off[q] = xyz_support(l1, l2, xo, nw, L, fx);
+          int rw;
+          if (chan == 0) rw = support_row(xo, l1, l2, nw, fx);
+          else if (chan == 1) rw = support_row(l1, xo, l2, nw, fx);
+          else rw = support_row(l1, l2, xo, nw, fx);
+          off[q] = rw * row4;
           if (fx) fl |= 4 << q;
         }
         if (k < 4) {
@@ -320,14 +658,13 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
           for (int q = 0; q < 4; ++q) { s_off[d * 4 + q] = off[q]; s_w[d * 4 + q] = w[q]; }
           s_fl[d] = fl;
         } else {
-          // getValueLocal (XYZVertexTwoParticle.hpp:355-376): site 0
 #pragma unroll
           for (int cc = 0; cc < 4; ++cc) {
             double v = 0.0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const double sg = (cc == kD && ((fl >> (2 + q)) & 1)) ? -w[q] : w[q];
-              v = fma(sg, gload(rG, (off[q] + cc * np) * 8), v);
+              v = fma(sg, gload(rG, (off[q] + cc) * 8), v);
             }
             s_loc[(e * 4 + (k - 4)) * 4 + cc] = v;
           }
@@ -335,9 +672,8 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
         if (k == 0) s_c[e] = Wc[e0 + e];
       }
       __syncthreads();
-      // (B) bundles: (point, buffer, site) -> 4 components into LDS
       for (int q = tid; q < nvalid * 4 * L; q += kXyzThreads) {
-        const int d = q / L, j = q - (q / L) * L;  // d = e * 4 + k
+        const int d = q / L, j = q - (q / L) * L;
         const int fl = s_fl[d];
         const int *cofs = (fl & 1) ? s_ci : s_cf;
         int off[4];
@@ -359,30 +695,16 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
         for (int cc = 0; cc < 4; ++cc) dst[cc * L] = v[cc];
       }
       __syncthreads();
-      // (C) channel algebra for (point, site, pair)
       if (cactive && ce < nvalid) {
         const double cw = s_c[ce];
-        const double *Xb = s_A + (size_t)(ce * 4 + 2 * cpair) * 4 * L;      // first bundle of the pair
-        const double *Yb = s_A + (size_t)(ce * 4 + 2 * cpair + 1) * 4 * L;  // second bundle
+        const double *Xb = s_A + (size_t)(ce * 4 + 2 * cpair) * 4 * L;
+        const double *Yb = s_A + (size_t)(ce * 4 + 2 * cpair + 1) * 4 * L;
         double a[4], b[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) { a[k] = cw * Xb[k * L + cj]; b[k] = Yb[k * L + cj]; }
         if (ch != 1) {
-          // particle-particle (S, signs +) and crossed particle-hole (U, all -)
-          // ladders (XYZFrgCore.cpp:223-275, 419-470)
-          const double sx = ch == 0 ? 1.0 : -1.0;
-          acc[kX] = fma(sx, fma(a[kD], b[kX], a[kX] * b[kD]), acc[kX]);
-          acc[kX] = fma(-1.0, fma(a[kZ], b[kY], a[kY] * b[kZ]), acc[kX]);
-          acc[kY] = fma(sx, fma(a[kD], b[kY], a[kY] * b[kD]), acc[kY]);
-          acc[kY] = fma(-1.0, fma(a[kZ], b[kX], a[kX] * b[kZ]), acc[kY]);
-          acc[kZ] = fma(sx, fma(a[kD], b[kZ], a[kZ] * b[kD]), acc[kZ]);
-          acc[kZ] = fma(-1.0, fma(a[kX], b[kY], a[kY] * b[kX]), acc[kZ]);
-          acc[kD] = fma(sx, fma(a[kD], b[kD], fma(a[kX], b[kX], fma(a[kY], b[kY], a[kZ] * b[kZ]))), acc[kD]);
+          ladder(acc, a, b, ch == 0 ? 1.0 : -1.0);
         } else {
-          // lattice bubble of the first pair (XYZFrgCore.cpp:295-322): entry i
-          // of getOverlap(rid) pairs site rid1 of bundle 0 and rid2 of bundle 1,
-          // each spin component through the entry's transformation; the two
-          // pair threads of a site take halves of its entries
           const double *B0 = s_A + (size_t)(ce * 4) * 4 * L;
           const double *B1 = s_A + (size_t)(ce * 4 + 1) * 4 * L;
           const int ib = F.ovl_off[cj], ie = F.ovl_off[cj + 1];
@@ -399,25 +721,17 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
           }
 #pragma unroll
           for (int k = 0; k < 4; ++k) acc[k] = fma(c4, r[k], acc[k]);
-          // chalice with Cb (pair 0) / Cb2 (pair 1) and inverse chalice with
-          // ICa / ICa2 (XYZFrgCore.cpp:324-416)
           const double *lc = s_loc + (ce * 4 + 2 * cpair) * 4;
           const double *li = s_loc + (ce * 4 + 2 * cpair + 1) * 4;
-          double Lc[4], Li[4];
+          double bw[4];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) { Lc[k] = lc[k]; Li[k] = li[k]; }
-          const double bw[4] = {cw * b[0], cw * b[1], cw * b[2], cw * b[3]};
-#pragma unroll
-          for (int k = 0; k < 3; ++k) acc[k] = fma(a[k], chal(Lc, k), fma(bw[k], chal(Li, k), acc[k]));
-          const double sc = (Lc[kD] + Lc[kX]) + (Lc[kY] + Lc[kZ]);
-          const double si = (Li[kD] + Li[kX]) + (Li[kY] + Li[kZ]);
-          acc[kD] = fma(-a[kD], sc, fma(-bw[kD], si, acc[kD]));
+          for (int k = 0; k < 4; ++k) bw[k] = cw * b[k];
+          chalices(acc, a, bw, lc, li);
         }
       }
       __syncthreads();
     }
   }
-  // reduce the (point, pair) partial sums per (component, site); prefactor 1/(2 pi)
   double *red = s_A;
   if (cactive) {
     const int g = ce * 2 + cpair;
@@ -431,7 +745,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
     for (int g = 0; g < ng; ++g) v += red[(size_t)g * 4 * L + q];
     v *= (1.0 / kTwoPi);
     const int c = q / L, j = q - (q / L) * L;
-    F.dG[(size_t)c * F.np + (size_t)item * L + j] = v;
+    F.dG[((size_t)item * L + j) * 4 + c] = v;
     if (isnan(v)) atomicOr(&F.flags[0], 1);
   }
 }
@@ -457,7 +771,6 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_corr_partial(XyzDev F) {
   const int n = *F.Cn;
   if (o >= n) return;
   const int nw = F.nw, L = F.L;
-  const long long np = F.np;
   const double *m = F.mesh, *v2 = F.v2;
   const double *G = F.G;
   const int slots = kXyzThreads / L;
@@ -487,17 +800,16 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_corr_partial(XyzDev F) {
         interp(m, nw, c, lU, uU, bU);
         const int srid = exch ? F.inv_rid[rid] : F.site_rid[rid];
         const int *perm = exch ? F.inv_perm + 3 * rid : F.site_perm + 3 * rid;
-        const double *P[4] = {G + (size_t)perm[0] * np + srid, G + (size_t)perm[1] * np + srid,
-                              G + (size_t)perm[2] * np + srid, G + (size_t)kD * np + srid};
+        const int comp[4] = {perm[0], perm[1], perm[2], kD};
         double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
           const int S = (i & 1) ? uS : lS, T = (i & 2) ? uT : lT, U = (i & 4) ? uU : lU;
           const double wi = ((i & 2) ? bT : (1 - bT)) * ((i & 1) ? bS : (1 - bS)) * ((i & 4) ? bU : (1 - bU));
           const bool fx = S < U;
-          const size_t row = ((size_t)(fx ? tri(U, S) : tri(S, U)) * nw + T) * L;
+          const double *el = G + (((size_t)(fx ? tri(U, S) : tri(S, U)) * nw + T) * L + srid) * 4;
 #pragma unroll
-          for (int k = 0; k < 4; ++k) v[k] = fma((k == kD && fx) ? -wi : wi, P[k][row], v[k]);
+          for (int k = 0; k < 4; ++k) v[k] = fma((k == kD && fx) ? -wi : wi, el[comp[k]], v[k]);
         }
         r[kX] = -v[kX];
         r[kY] = -v[kY];
@@ -508,7 +820,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_corr_partial(XyzDev F) {
         // egg: getValue(0, 0, w + wp + nu, w - wp, nu, c, None)
         double vv[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) vv[k] = xyz_value_none(G, np, m, nw, L, 0, 0, w + wp + nu, w - wp, nu, k);
+        for (int k = 0; k < 4; ++k) vv[k] = xyz_value_none(G, m, nw, L, 0, 0, w + wp + nu, w - wp, nu, k);
         r[kX] += 0.5 * (vv[kX] - vv[kY] - vv[kZ] + vv[kD]);
         r[kY] += 0.5 * (-vv[kX] + vv[kY] - vv[kZ] + vv[kD]);
         r[kZ] += 0.5 * (-vv[kX] - vv[kY] + vv[kZ] + vv[kD]);
@@ -569,8 +881,10 @@ struct sp_xyz_ctx {
   XyzDev fd{};
   double cutoff = 0.0;
   bool have_flow = false;
+  bool direct = false;  // bubble summed per point (k_xyz_dgamma_direct)
   size_t lds = 0;
   std::vector<void *> allocs;
+  std::vector<double> host;  // plane <-> interleaved staging of the state
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[4];
   double prof_ms[4] = {0, 0, 0, 0};
@@ -664,9 +978,22 @@ hipError_t xyz_launch_compute(sp_xyz_ctx *ctx) {
   if (e != hipSuccess) return e;
   XyzProf ps(ctx, 0);
   const long long rounds = (F.nf + 8 * kXyzChunk - 1) / (8 * kXyzChunk);
-  hipLaunchKernelGGL(k_xyz_dgamma, dim3((unsigned)(rounds * 8 * kXyzChunk)), dim3(kXyzThreads), ctx->lds, ctx->stream,
-                     F);
+  const dim3 grid((unsigned)(rounds * 8 * kXyzChunk));
+  if (ctx->direct)
+    hipLaunchKernelGGL(k_xyz_dgamma_direct, grid, dim3(kXyzThreads), ctx->lds, ctx->stream, F);
+  else
+    hipLaunchKernelGGL(k_xyz_dgamma, grid, dim3(kXyzThreads), ctx->lds, ctx->stream, F);
   return hipGetLastError();
+}
+
+// the reference's four planes X, Y, Z, D <-> the interleaved device layout
+void planes_to_interleaved(const double *planes, double *out, size_t np) {
+  for (size_t i = 0; i < np; ++i)
+    for (int c = 0; c < 4; ++c) out[4 * i + c] = planes[c * np + i];
+}
+void interleaved_to_planes(const double *in, double *planes, size_t np) {
+  for (size_t i = 0; i < np; ++i)
+    for (int c = 0; c < 4; ++c) planes[c * np + i] = in[4 * i + c];
 }
 
 }  // namespace
@@ -683,6 +1010,7 @@ int sp_xyz_ctx_footprint(const sp_xyz_problem *p, long long *bytes) {
   b += 2 * (long long)p->nw * emax * 8 + 2 * emax * 8 + (p->nw + 1) * 4;      // weight tables
   b += emax * 4 * p->L * 8 + 4LL * p->B * p->Rb * 8 + 16;                     // correlation scratch, output, flags
   b += 4LL * (8 * p->L + 1 + 8 * O + 2 * p->R + 4 * p->B * p->Rb);            // lattice tables
+  b += 16LL * O;                                                              // bubble-matrix indices
   *bytes = b;
   return SP_OK;
 }
@@ -706,18 +1034,49 @@ int sp_xyz_ctx_create(const sp_xyz_problem *p, sp_xyz_ctx **out) {
   f.nf = (long long)p->nw * p->nw * (p->nw + 1) / 2;
   f.np = f.nf * p->L;
   f.emax = 2 * p->nw + 8;
-  int nb = kXyzThreads / (2 * p->L);
-  if (nb > kXyzMaxPoints) nb = kXyzMaxPoints;
-  if (nb < 1) nb = 1;
-  f.nb = nb;
-  ctx->lds = (size_t)xyz_lds(f.nw, f.L, f.nb).total;
+  const int O = p->ovl_off[p->L];
+  // bubble matrices: one per spin-component pair (a, b) that some overlap
+  // transformation maps a bundle component onto, plus the density pair
+  int slot_of[16];
+  for (int k = 0; k < 16; ++k) slot_of[k] = -1;
+  f.npairs = 0;
+  auto use_pair = [&](int a, int b) {
+    if (slot_of[4 * a + b] < 0) {
+      slot_of[4 * a + b] = f.npairs;
+      f.pair[f.npairs++] = 4 * a + b;
+    }
+  };
+  use_pair(kD, kD);
+  for (int i = 0; i < O; ++i)
+    for (int c = 0; c < 3; ++c) use_pair(p->ovl_perm1[3 * i + c], p->ovl_perm2[3 * i + c]);
+  const int Lp = p->L + (p->L & 1);
+  std::vector<int4> midx(O > 0 ? O : 1);
+  for (int i = 0; i < O; ++i) {
+    const int base = p->ovl_rid1[i] * Lp + p->ovl_rid2[i];
+    int v[4];
+    for (int c = 0; c < 3; ++c) v[c] = slot_of[4 * p->ovl_perm1[3 * i + c] + p->ovl_perm2[3 * i + c]] * Lp * Lp + base;
+    v[3] = slot_of[4 * kD + kD] * Lp * Lp + base;
+    midx[i] = make_int4(v[0], v[1], v[2], v[3]);
+  }
   int max_lds = 0;
   SP_XHIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  // rank-update kernel: (slot, rid) threads, one chunk = one point per slot
+  f.slots = kXyzThreads / p->L;
+  f.nb = f.slots;
+  ctx->lds = (size_t)xyz_lds(f.nw, f.L, f.nb, f.slots, f.npairs).total;
+  const char *force = std::getenv("SPFRG_XYZ_BUBBLE");
+  ctx->direct = ctx->lds > (size_t)max_lds || (force && std::strcmp(force, "direct") == 0);
+  if (ctx->direct) {
+    int nb = kXyzThreads / (2 * p->L);
+    if (nb > kXyzMaxPoints) nb = kXyzMaxPoints;
+    if (nb < 1) nb = 1;
+    f.nb = nb;
+    ctx->lds = (size_t)xyz_lds_direct(f.nw, f.L, f.nb).total;
+  }
   if (ctx->lds > (size_t)max_lds) return xfail(ctx, SP_EARG, "problem too large for the LDS plan of the dGamma kernel");
   // 32-bit byte offsets of the buffer loads cover all four planes
   if (4 * f.np * 8 >= (1LL << 31)) return xfail(ctx, SP_EARG, "vertex too large for 32-bit byte offsets");
   f.g_bytes = (unsigned)(4 * f.np * 8);
-  const int O = p->ovl_off[p->L];
   const size_t nv = 4 * (size_t)f.np;
   const int emax = f.emax;
   const size_t nc = (size_t)f.B * f.Rb;
@@ -735,6 +1094,10 @@ int sp_xyz_ctx_create(const sp_xyz_problem *p, sp_xyz_ctx **out) {
       (rc = xalloc(ctx, &f.Cn, 1)) || (rc = xalloc(ctx, &f.corr_partial, (size_t)emax * 4 * f.L)) ||
       (rc = xalloc(ctx, &f.corr_out, 4 * nc)) || (rc = xalloc(ctx, &f.flags, 4)))
     return rc;
+  int4 *midx_d = nullptr;
+  if ((rc = xalloc(ctx, &midx_d, midx.size()))) return rc;
+  SP_XHIP(hipMemcpy(midx_d, midx.data(), midx.size() * sizeof(int4), hipMemcpyHostToDevice));
+  f.ovl_midx = midx_d;
   f.mesh = mesh;
   f.site_rid = site;
   f.site_perm = sitep;
@@ -775,6 +1138,8 @@ int sp_xyz_ctx_create(const sp_xyz_problem *p, sp_xyz_ctx **out) {
   SP_XHIP(hipMemset(f.Wn, 0, f.nw * 4));
   SP_XHIP(hipMemset(f.Cn, 0, 4));
   SP_XHIP(hipFuncSetAttribute((const void *)k_xyz_dgamma, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_XHIP(hipFuncSetAttribute((const void *)k_xyz_dgamma_direct, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  ctx->host.resize(nv);
   return SP_OK;
 }
 
@@ -794,7 +1159,8 @@ int sp_xyz_set_state(sp_xyz_ctx *ctx, double cutoff, const double *v2, const dou
   if (!std::isfinite(cutoff) || !(cutoff > 0.0)) return xfail(ctx, SP_EARG, "cutoff must be finite and positive");
   SP_XHIP(hipStreamSynchronize(ctx->stream));
   SP_XHIP(hipMemcpy(ctx->fd.v2, v2, ctx->fd.nw * 8, hipMemcpyHostToDevice));
-  SP_XHIP(hipMemcpy(ctx->fd.G, v4, 4 * (size_t)ctx->fd.np * 8, hipMemcpyHostToDevice));
+  planes_to_interleaved(v4, ctx->host.data(), (size_t)ctx->fd.np);
+  SP_XHIP(hipMemcpy(ctx->fd.G, ctx->host.data(), 4 * (size_t)ctx->fd.np * 8, hipMemcpyHostToDevice));
   SP_XHIP(hipMemset(ctx->fd.flags, 0, 16));
   ctx->cutoff = cutoff;
   ctx->have_flow = false;
@@ -806,7 +1172,10 @@ int sp_xyz_get_state(sp_xyz_ctx *ctx, double *cutoff, double *v2, double *v4) {
   SP_XHIP(hipStreamSynchronize(ctx->stream));
   if (cutoff) *cutoff = ctx->cutoff;
   if (v2) SP_XHIP(hipMemcpy(v2, ctx->fd.v2, ctx->fd.nw * 8, hipMemcpyDeviceToHost));
-  if (v4) SP_XHIP(hipMemcpy(v4, ctx->fd.G, 4 * (size_t)ctx->fd.np * 8, hipMemcpyDeviceToHost));
+  if (v4) {
+    SP_XHIP(hipMemcpy(ctx->host.data(), ctx->fd.G, 4 * (size_t)ctx->fd.np * 8, hipMemcpyDeviceToHost));
+    interleaved_to_planes(ctx->host.data(), v4, (size_t)ctx->fd.np);
+  }
   return SP_OK;
 }
 
@@ -815,7 +1184,10 @@ int sp_xyz_get_flow(sp_xyz_ctx *ctx, double *dv2, double *dv4) {
   if (!ctx->have_flow) return xfail(ctx, SP_ESTATE, "no flow computed at the current cutoff");
   SP_XHIP(hipStreamSynchronize(ctx->stream));
   if (dv2) SP_XHIP(hipMemcpy(dv2, ctx->fd.dv2, ctx->fd.nw * 8, hipMemcpyDeviceToHost));
-  if (dv4) SP_XHIP(hipMemcpy(dv4, ctx->fd.dG, 4 * (size_t)ctx->fd.np * 8, hipMemcpyDeviceToHost));
+  if (dv4) {
+    SP_XHIP(hipMemcpy(ctx->host.data(), ctx->fd.dG, 4 * (size_t)ctx->fd.np * 8, hipMemcpyDeviceToHost));
+    interleaved_to_planes(ctx->host.data(), dv4, (size_t)ctx->fd.np);
+  }
   return SP_OK;
 }
 

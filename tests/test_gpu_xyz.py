@@ -10,7 +10,10 @@ range 3, 14 frequencies; x <-> y spin permutations on inverted sites and
 overlaps) at oracle-advanced states, and the honeycomb Kitaev-Gamma task run
 with the XYZ core (its off-diagonal Gamma couplings are ignored by the XYZ
 initial condition, as in the reference; all six spin permutations occur among
-its overlap transformations) at frequency-dependent states.
+its overlap transformations) at frequency-dependent states.  Both dGamma
+kernels are covered: the default one (lattice bubble as rank updates of
+per-pair matrices in LDS) and the direct one used when those matrices exceed
+the LDS of a workgroup (forced here with SPFRG_XYZ_BUBBLE=direct).
 """
 import sys
 
@@ -103,9 +106,12 @@ def test_xyz_full_flow_golden(ref2):
     assert worst_g < 1e-5
 
 
-def test_xyz_permuted_honeycomb_parity():
+@pytest.mark.parametrize("bubble", ["rank-update", "direct"])
+def test_xyz_permuted_honeycomb_parity(bubble, monkeypatch):
     """Honeycomb Kitaev-Gamma range 3 with the XYZ core: every spin permutation
     among the overlap transformations; frequency-dependent states."""
+    if bubble == "direct":
+        monkeypatch.setenv("SPFRG_XYZ_BUBBLE", "direct")
     from spinparser_amd.problem import xyz_problem
     from spinparser_amd.task import parse_task
     from oracle import XyzOracle
@@ -117,4 +123,5 @@ def test_xyz_permuted_honeycomb_parity():
     for k in (0, 10, 30):
         cut, v2, v4 = _advanced_state(o, p, min(k, 20))
         cut = float(p.cutoffs[k])
-        _compare(p, o, cut, v2, v4, "XYZ honeycomb-kitaev-gamma r3 nw=%d L=%d O=%d" % (p.nw, p.L, p.overlap_total))
+        _compare(p, o, cut, v2, v4, "XYZ honeycomb-kitaev-gamma r3 nw=%d L=%d O=%d (%s)"
+                 % (p.nw, p.L, p.overlap_total, bubble))
