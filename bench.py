@@ -110,6 +110,8 @@ TRAFFIC_KIB = {
                                     "fetch_scale": 2.00, "write_scale": 1.00},
     ("kagome-dm-r5-nw48", "f64"): {"fetch_kib": 1304870.0, "write_kib": 134064.0,
                                    "fetch_scale": 2.00, "write_scale": 1.00},
+    ("honeycomb-kitaev-r8-nw64", "f64"): {"fetch_kib": 3260340.0, "write_kib": 95720.9,
+                                          "fetch_scale": 2.00, "write_scale": 1.00},
 }
 
 
