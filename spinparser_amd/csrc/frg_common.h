@@ -207,6 +207,27 @@ __device__ __forceinline__ Interp interp_of(const double *m, int nw, double x) {
   return r;
 }
 
+// two interpolateOffset calls whose bracket searches run in one binary-lifting
+// loop, so that the two chains of dependent mesh reads overlap; each result
+// equals interp_of(m, n, x)
+__device__ __forceinline__ void interp_pair(const double *m, int n, double x0, double x1, Interp &a, Interp &b) {
+  int p0 = 0, p1 = 0;
+  for (int step = 1 << (31 - __clz(n)); step > 0; step >>= 1) {
+    const int q0 = p0 + step, q1 = p1 + step;
+    if (q0 <= n && m[q0 - 1] <= x0) p0 = q0;
+    if (q1 <= n && m[q1 - 1] <= x1) p1 = q1;
+  }
+  auto finish = [&](Interp &r, int c, double w) {
+    if (!(w > m[0])) { r.lo = 0; r.hi = 0; r.b = 0.0; return; }
+    if (c >= n) { r.lo = n - 1; r.hi = n - 1; r.b = 0.0; return; }
+    r.lo = c - 1;
+    r.hi = c;
+    r.b = (w - m[r.lo]) / (m[r.hi] - m[r.lo]);
+  };
+  finish(a, p0, x0);
+  finish(b, p1, x1);
+}
+
 template <class V2> __device__ __forceinline__ V2 buf_load(__amdgpu_buffer_rsrc_t r, int byte_off);
 
 template <> __device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, int byte_off) {
@@ -233,6 +254,7 @@ struct Desc4 {
   int off[4];
   double ws[4];
   double wd[4];
+  int neg;  // bit q: support q is stored s <-> u exchanged (density weight negated)
 };
 
 __device__ __forceinline__ Desc4 assemble(int chan, int ex, Interp i1, Interp i2, int nw, int rowbytes) {
@@ -241,6 +263,7 @@ __device__ __forceinline__ Desc4 assemble(int chan, int ex, Interp i1, Interp i2
   const int l2[4] = {i2.lo, i2.lo, i2.hi, i2.hi};
   const double b1 = i1.b, b2 = i2.b;
   const double w[4] = {(1 - b2) * (1 - b1), (1 - b2) * b1, b2 * (1 - b1), b2 * b1};
+  d.neg = 0;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     int so, to, uo;
@@ -252,6 +275,7 @@ __device__ __forceinline__ Desc4 assemble(int chan, int ex, Interp i1, Interp i2
     d.off[q] = row * rowbytes;
     d.ws[q] = w[q];
     d.wd[q] = neg ? -w[q] : w[q];
+    if (neg) d.neg |= 1 << q;
   }
   return d;
 }

@@ -21,16 +21,19 @@
 //   k_dsigma   one wavefront per (flow, mesh point); lanes split the range sum.
 //   k_weights  one thread per (flow, exact index): window points, coefficients.
 //   k_dgamma   one 256-thread workgroup per (flow, frequency triple (s,t,u)),
-//              blocks mapped to XCDs in contiguous item ranges.  Threads are
-//              (slot, rid) pairs; per batch of evaluation points:
-//              (A) one thread per (point, pair of access buffers): the two
-//                  bracket searches a pair shares, sign / site-exchange folding,
-//                  support byte offsets and weights -> LDS;
-//              (B) per (slot, rid): 4 x 4 vertex gathers (buffer loads with
-//                  32-bit offsets) and the channel algebra;
+//              blocks mapped to XCDs in contiguous item ranges.  The evaluation
+//              points of the three channels form one sequence of chunks; per chunk:
+//              (A) producer wave (wave 3), one lane per (point, pair of access
+//                  buffers): the two bracket searches a pair shares, sign /
+//                  site-exchange folding, support byte offsets and weights ->
+//                  one of two LDS descriptor buffers, one chunk ahead;
+//              (B) worker waves 0..2, (slot, rid) pairs: 4 x 4 vertex gathers
+//                  (buffer loads with 32-bit offsets) and the channel algebra
+//                  from the other descriptor buffer;
 //              (C, t channel) the RPA lattice bubble accumulated as
 //                  C[a][b] += sum_e c_e A2_e[a] A3_e[b] in LDS (2x2 register
-//                  tiles), contracted with the overlap lists once per item.
+//                  tiles, all waves), contracted with the overlap lists once
+//                  per item.
 //              Template MIXED: FP32 gathers and algebra with compensated
 //              (two-sum) accumulation; descriptors, frequencies and the result
 //              stay FP64.
@@ -66,8 +69,13 @@ namespace {
 #ifndef SPFRG_THREADS
 #define SPFRG_THREADS 256
 #endif
+// evaluation points per chunk (FP64 / mixed form): the descriptor buffers and
+// bubble factors of a chunk must leave LDS for 4 workgroups per CU
 #ifndef SPFRG_MAX_BATCH
-#define SPFRG_MAX_BATCH 32
+#define SPFRG_MAX_BATCH 16
+#endif
+#ifndef SPFRG_MAX_BATCH_MIXED
+#define SPFRG_MAX_BATCH_MIXED 32
 #endif
 #ifndef SPFRG_MIN_BLOCKS
 #define SPFRG_MIN_BLOCKS 4
@@ -77,6 +85,7 @@ namespace {
 #endif
 constexpr int kThreads = SPFRG_THREADS;
 constexpr int kMaxBatch = SPFRG_MAX_BATCH;
+constexpr int kMaxBatchMixed = SPFRG_MAX_BATCH_MIXED;
 #ifndef SPFRG_UNROLL_FP64
 #define SPFRG_UNROLL_FP64 1
 #endif
@@ -226,9 +235,11 @@ __global__ __launch_bounds__(64) void k_dsigma(const FlowDev *__restrict__ fl) {
 // ---------------------------------------------------------------------------
 // LDS carve-up (bytes), computed identically on host and device.  Lp = L
 // rounded up to even (2x2 register tiles of the lattice-bubble update); vs =
-// size of the gather type (8 FP64, 4 mixed).
+// size of the gather type (8 FP64, 4 mixed).  The per-chunk descriptors (c,
+// off, w, ex, loc) are double-buffered: the producer wave fills one buffer
+// while the worker waves read the other.
 struct LdsLayout {
-  int mesh, c, off, ws, wd, ex, loc, P, Q, C, rpa, red, site, inv, total;
+  int mesh, c, off, w, ex, loc, dbuf, P, Q, C, rpa, red, site, inv, total;  // buffer b at + b * dbuf
 };
 
 __host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots, int vs) {
@@ -237,12 +248,14 @@ __host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots
   int off = 0;
   auto take = [&](int bytes) { int o = off; off += (bytes + 15) & ~15; return o; };
   l.mesh = take(nw * 8);
+  const int d0 = off;
   l.c = take(nb * 8);
   l.off = take(nb * 4 * 16);
-  l.ws = take(nb * 4 * 4 * vs);
-  l.wd = take(nb * 4 * 4 * vs);
+  l.w = take(nb * 4 * 4 * vs);
   l.ex = take(nb * 4 * 4);
   l.loc = take(nb * 4 * 2 * vs);
+  l.dbuf = off - d0;
+  off += l.dbuf;  // second descriptor buffer
   const int pq = 2 * nb * Lp * vs;
   const int red = slots * L * 2 * 8;
   l.P = take(pq > red ? pq : red);  // the final slot reduction reuses P
@@ -275,6 +288,12 @@ __device__ __forceinline__ void ksum_add(KSum &k, float x) {
   k.s = t;
 }
 
+// dGamma kernel.  Waves 0..2 are workers, (slot, rid) pairs with slots =
+// floor(192 / L); wave 3 is the producer.  The evaluation points of the three
+// channels form one sequence of chunks of nb points; while the workers gather
+// and evaluate chunk j, the producer builds the descriptors of chunk j + 1 into
+// the other LDS buffer, so the bracket searches and the t channel's site-local
+// gathers overlap the workers' gathers instead of stalling them.
 template <bool MIXED>
 __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOCKS) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,
                                                      long long per_xcd) {
@@ -297,12 +316,6 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
   const int Lp = L + (L & 1);
   const LdsLayout lay = lds_layout(nw, L, nb, slots, (int)sizeof(V));
   double *s_mesh = reinterpret_cast<double *>(lds_raw + lay.mesh);
-  double *s_c = reinterpret_cast<double *>(lds_raw + lay.c);
-  int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off);
-  V2 *s_ws = reinterpret_cast<V2 *>(lds_raw + lay.ws);
-  V2 *s_wd = reinterpret_cast<V2 *>(lds_raw + lay.wd);
-  int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex);
-  V *s_loc = reinterpret_cast<V *>(lds_raw + lay.loc);
   V *s_P = reinterpret_cast<V *>(lds_raw + lay.P);
   V *s_Q = reinterpret_cast<V *>(lds_raw + lay.Q);
   double *s_C = reinterpret_cast<double *>(lds_raw + lay.C);  // FP64 sums, or {sum, comp} float pairs
@@ -330,6 +343,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
   for (int i = tid; i < L; i += kThreads) { s_site[i] = F.site_rid[i]; s_inv[i] = F.inv_rid[i]; }
   for (int i = tid; i < 2 * Lp * Lp; i += kThreads) s_C[i] = 0.0;
   for (int i = tid; i < 2 * nb * Lp; i += kThreads) { s_P[i] = V(0); s_Q[i] = V(0); }
+  for (int i = tid; i < 2 * L; i += kThreads) s_rpa[i] = 0.0;
   __syncthreads();
 
   const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
@@ -338,282 +352,309 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
   const double w2p = 0.5 * (s - t - u);
   const double w2 = 0.5 * (s + t - u);
 
+  const int lane = tid & 63;
+  const bool producer = tid >= kThreads - 64;
   const int slot = tid / L;
   const int rid = tid - slot * L;
-  const bool active = slot < slots;
+  const bool active = slot < slots;  // slots * L <= kThreads - 64: workers only
   const int sr_fwd = active ? s_site[rid] * esz : 0;
   const int sr_inv = active ? s_inv[rid] * esz : 0;
   double acc_s = 0.0, acc_d = 0.0;   // FP64 form
   KSum ks{0.f, 0.f}, kd{0.f, 0.f};  // mixed form
 
-  for (int ch = 0; ch < 3; ++ch) {
-    const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
-    const int n = F.Wn[xo];
-    const double *Wwp = F.Wwp + (size_t)xo * F.emax;
-    const double *Wc = F.Wc + (size_t)xo * F.emax;
-    const double sgn = ch == 2 ? -1.0 : 1.0;
-    for (int e0 = 0; e0 < n; e0 += nb) {
-      const int nvalid = n - e0 < nb ? n - e0 : nb;
-      // (A) one thread per (point, pair): the two bracket searches shared by
-      // the 2 (s, u channels) or 4 (t channel) access buffers of the pair
-      if (tid < 2 * nvalid) {
-        const int e = tid >> 1, pr = tid & 1;
-        const double wp = Wwp[e0 + e];
-        double x0, x1;
-        PairBuf pb[4];
-        int nbufs = 2;
-        if (ch == 0) {
-          if (pr == 0) {
-            x0 = fabs(w1 + wp); x1 = fabs(w2 + wp);
-            pb[0] = {0, s, -w2 - wp, false, 0};
-            pb[1] = {2, s, w1 + wp, true, 0};
-          } else {
-            x0 = fabs(w1p + wp); x1 = fabs(w2p + wp);
-            pb[0] = {1, s, -w2p - wp, false, 0};
-            pb[1] = {3, s, w1p + wp, true, 0};
-          }
-        } else if (ch == 1) {
-          nbufs = 4;
-          if (pr == 0) {
-            x0 = fabs(w1 - wp); x1 = fabs(w1p + wp);
-            pb[0] = {0, w1 - wp, w1p + wp, false, 1};
-            pb[1] = {2, w1p + wp, w1 - wp, true, 1};
-            pb[2] = {5, w1 - wp, -t, false, 2};
-            pb[3] = {7, w1p + wp, -t, true, 2};
-          } else {
-            x0 = fabs(w2p - wp); x1 = fabs(w2 + wp);
-            pb[0] = {1, w2p - wp, -w2 - wp, false, 1};
-            pb[1] = {3, w2 + wp, wp - w2p, true, 1};
-            pb[2] = {4, w2p - wp, t, false, 2};
-            pb[3] = {6, w2 + wp, t, true, 2};
-          }
-        } else {
-          if (pr == 0) {
-            x0 = fabs(w1 + wp); x1 = fabs(w2p - wp);
-            pb[0] = {0, w1 + wp, u, false, 2};
-            pb[1] = {2, w2p - wp, u, true, 2};
-          } else {
-            x0 = fabs(w1p + wp); x1 = fabs(w2 - wp);
-            pb[0] = {1, w1p + wp, u, false, 2};
-            pb[1] = {3, w2 - wp, u, true, 2};
-          }
-        }
-        const Interp i0 = interp_of(s_mesh, nw, x0);
-        const Interp i1 = interp_of(s_mesh, nw, x1);
-        // fully unrolled with a guard so that pb[] stays in registers (a
-        // dynamically indexed local array would live in scratch memory)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (q >= nbufs) break;
-          const PairBuf &b = pb[q];
-          const Desc4 d = assemble(b.chan, xo, b.swap ? i1 : i0, b.swap ? i0 : i1, nw, rowbytes);
-          if (b.k < 4) {
-            const int sk = e * 4 + b.k;
-            s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
-            s_ws[2 * sk] = mk2<V2>(V(d.ws[0]), V(d.ws[1]));
-            s_ws[2 * sk + 1] = mk2<V2>(V(d.ws[2]), V(d.ws[3]));
-            s_wd[2 * sk] = mk2<V2>(V(d.wd[0]), V(d.wd[1]));
-            s_wd[2 * sk + 1] = mk2<V2>(V(d.wd[2]), V(d.wd[3]));
-            s_ex[sk] = ((b.a < 0) != (b.c < 0)) ? 1 : 0;
-          } else {
-            // SU2VertexTwoParticle::getValueLocal (:347-360): site 0, no exchange
-            V vs = V(0), vd = V(0);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const V2 v = buf_load<V2>(rG, d.off[i]);
-              vs = fma(V(d.ws[i]), v.x, vs);
-              vd = fma(V(d.wd[i]), v.y, vd);
+  // chunk schedule: chunk j covers points [e0, e0 + nb) of channel ch
+  const int nS = F.Wn[so], nT = F.Wn[to], nU = F.Wn[uo];
+  const int cS = (nS + nb - 1) / nb, cT = (nT + nb - 1) / nb, cU = (nU + nb - 1) / nb;
+  const int nchunks = cS + cT + cU;
+
+  // (A) producer: one lane per (point, pair) of chunk jj builds the descriptors
+  // into buffer bf -- the two bracket searches shared by the 2 (s, u channels)
+  // or 4 (t channel) access buffers of the pair; the t channel's site-local
+  // buffers are gathered right away
+  // iteration jc: the producer builds chunk jc + 1 while the workers evaluate
+  // chunk jc (jc = -1: the first chunk's descriptors only)
+  for (int jc = -1; jc < nchunks; ++jc) {
+    const int cur = jc & 1;
+    const int ch = jc < cS ? 0 : (jc < cS + cT ? 1 : 2);
+    const int e0 = (jc - (ch == 0 ? 0 : (ch == 1 ? cS : cS + cT))) * nb;
+    const int n = ch == 0 ? nS : (ch == 1 ? nT : nU);
+    const int nvalid = n - e0 < nb ? n - e0 : nb;
+    if (producer) {
+      const int jj = jc + 1, bf = jj & 1;
+      if (jj < nchunks) {
+        const int ch = jj < cS ? 0 : (jj < cS + cT ? 1 : 2);
+        const int e0 = (jj - (ch == 0 ? 0 : (ch == 1 ? cS : cS + cT))) * nb;
+        const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
+        const int n = ch == 0 ? nS : (ch == 1 ? nT : nU);
+        const int nvalid = n - e0 < nb ? n - e0 : nb;
+        if (lane < 2 * nvalid) {
+          double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
+          int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
+          V2 *s_w = reinterpret_cast<V2 *>(lds_raw + lay.w + bf * lay.dbuf);
+          int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex + bf * lay.dbuf);
+          V *s_loc = reinterpret_cast<V *>(lds_raw + lay.loc + bf * lay.dbuf);
+          const int e = lane >> 1, pr = lane & 1;
+          const double wp = F.Wwp[(size_t)xo * F.emax + e0 + e];
+          double x0, x1;
+          PairBuf pb[4];
+          int nbufs = 2;
+          if (ch == 0) {
+            if (pr == 0) {
+              x0 = fabs(w1 + wp); x1 = fabs(w2 + wp);
+              pb[0] = {0, s, -w2 - wp, false, 0};
+              pb[1] = {2, s, w1 + wp, true, 0};
+            } else {
+              x0 = fabs(w1p + wp); x1 = fabs(w2p + wp);
+              pb[0] = {1, s, -w2p - wp, false, 0};
+              pb[1] = {3, s, w1p + wp, true, 0};
             }
-            s_loc[(e * 4 + (b.k - 4)) * 2 + 0] = vs;
-            s_loc[(e * 4 + (b.k - 4)) * 2 + 1] = vd;
+          } else if (ch == 1) {
+            nbufs = 4;
+            if (pr == 0) {
+              x0 = fabs(w1 - wp); x1 = fabs(w1p + wp);
+              pb[0] = {0, w1 - wp, w1p + wp, false, 1};
+              pb[1] = {2, w1p + wp, w1 - wp, true, 1};
+              pb[2] = {5, w1 - wp, -t, false, 2};
+              pb[3] = {7, w1p + wp, -t, true, 2};
+            } else {
+              x0 = fabs(w2p - wp); x1 = fabs(w2 + wp);
+              pb[0] = {1, w2p - wp, -w2 - wp, false, 1};
+              pb[1] = {3, w2 + wp, wp - w2p, true, 1};
+              pb[2] = {4, w2p - wp, t, false, 2};
+              pb[3] = {6, w2 + wp, t, true, 2};
+            }
+          } else {
+            if (pr == 0) {
+              x0 = fabs(w1 + wp); x1 = fabs(w2p - wp);
+              pb[0] = {0, w1 + wp, u, false, 2};
+              pb[1] = {2, w2p - wp, u, true, 2};
+            } else {
+              x0 = fabs(w1p + wp); x1 = fabs(w2 - wp);
+              pb[0] = {1, w1p + wp, u, false, 2};
+              pb[1] = {3, w2 - wp, u, true, 2};
+            }
           }
+          Interp i0, i1;
+          interp_pair(s_mesh, nw, x0, x1, i0, i1);
+          // fully unrolled with a guard so that pb[] stays in registers (a
+          // dynamically indexed local array would live in scratch memory)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (q >= nbufs) break;
+            const PairBuf &bb = pb[q];
+            const Desc4 d = assemble(bb.chan, xo, bb.swap ? i1 : i0, bb.swap ? i0 : i1, nw, rowbytes);
+            if (bb.k < 4) {
+              const int sk = e * 4 + bb.k;
+              s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
+              s_w[2 * sk] = mk2<V2>(V(d.ws[0]), V(d.ws[1]));
+              s_w[2 * sk + 1] = mk2<V2>(V(d.ws[2]), V(d.ws[3]));
+              s_ex[sk] = (((bb.a < 0) != (bb.c < 0)) ? 1 : 0) | (d.neg << 1);
+            } else {
+              // SU2VertexTwoParticle::getValueLocal (:347-360): site 0, no exchange
+              V vs = V(0), vd = V(0);
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const V2 v = buf_load<V2>(rG, d.off[i]);
+                vs = fma(V(d.ws[i]), v.x, vs);
+                vd = fma(V(d.wd[i]), v.y, vd);
+              }
+              s_loc[(e * 4 + (bb.k - 4)) * 2 + 0] = vs;
+              s_loc[(e * 4 + (bb.k - 4)) * 2 + 1] = vd;
+            }
+          }
+          if (pr == 0) s_c[e] = (ch == 2 ? -1.0 : 1.0) * F.Wc[(size_t)xo * F.emax + e0 + e];
         }
-        if (pr == 0) s_c[e] = sgn * Wc[e0 + e];
       }
-      __syncthreads();
+    } else if (active && jc >= 0) {
       // (B) gathers and channel algebra per (slot, rid); U evaluation points
       // per iteration so that 16 U loads are in flight before the first use
-      if (active) {
-        constexpr int U = MIXED ? SPFRG_UNROLL_MIXED : SPFRG_UNROLL_FP64;
-        for (int e = slot; e < nvalid; e += U * slots) {
-          int ev[U];
-          bool ok[U];
-          // U == 1: loads and weights per access buffer, leaving the schedule
-          // to the compiler; U > 1: all 16 U loads issued before any use
-          constexpr int kGather = MIXED ? SPFRG_GATHER_MIXED : SPFRG_GATHER;
-          constexpr bool kPre = U > 1 || kGather == 2;
-          V2 val[kPre ? U : 1][16];
+      const double *s_c = reinterpret_cast<const double *>(lds_raw + lay.c + cur * lay.dbuf);
+      const int4 *s_off = reinterpret_cast<const int4 *>(lds_raw + lay.off + cur * lay.dbuf);
+      const V2 *s_w = reinterpret_cast<const V2 *>(lds_raw + lay.w + cur * lay.dbuf);
+      const int *s_ex = reinterpret_cast<const int *>(lds_raw + lay.ex + cur * lay.dbuf);
+      const V *s_loc = reinterpret_cast<const V *>(lds_raw + lay.loc + cur * lay.dbuf);
+      constexpr int U = MIXED ? SPFRG_UNROLL_MIXED : SPFRG_UNROLL_FP64;
+      for (int e = slot; e < nvalid; e += U * slots) {
+        int ev[U];
+        bool ok[U];
+        // U == 1: loads and weights per access buffer, leaving the schedule
+        // to the compiler; U > 1: all 16 U loads issued before any use
+        constexpr int kGather = MIXED ? SPFRG_GATHER_MIXED : SPFRG_GATHER;
+        constexpr bool kPre = U > 1 || kGather == 2;
+        V2 val[kPre ? U : 1][16];
 #pragma unroll
-          for (int q = 0; q < U; ++q) {
-            ev[q] = e + q * slots;
-            ok[q] = ev[q] < nvalid;
-            if (!ok[q]) ev[q] = e;
-            if (kPre) {
-#pragma unroll
-              for (int k = 0; k < 4; ++k) {
-                const int sk = ev[q] * 4 + k;
-                const int sr = s_ex[sk] ? sr_inv : sr_fwd;
-                const int4 o = s_off[sk];
-                val[q][4 * k + 0] = buf_load<V2>(rG, o.x + sr);
-                val[q][4 * k + 1] = buf_load<V2>(rG, o.y + sr);
-                val[q][4 * k + 2] = buf_load<V2>(rG, o.z + sr);
-                val[q][4 * k + 3] = buf_load<V2>(rG, o.w + sr);
-              }
-            }
-          }
-#pragma unroll
-          for (int q = 0; q < U; ++q) {
-            if (!ok[q]) continue;
-            const int ee = ev[q];
-            V As[4], Ad[4];
-            V2 vn[4];
-            if (!kPre && kGather == 1) {
-              const int sr = s_ex[ee * 4] ? sr_inv : sr_fwd;
-              const int4 o = s_off[ee * 4];
-              vn[0] = buf_load<V2>(rG, o.x + sr);
-              vn[1] = buf_load<V2>(rG, o.y + sr);
-              vn[2] = buf_load<V2>(rG, o.z + sr);
-              vn[3] = buf_load<V2>(rG, o.w + sr);
-            }
+        for (int q = 0; q < U; ++q) {
+          ev[q] = e + q * slots;
+          ok[q] = ev[q] < nvalid;
+          if (!ok[q]) ev[q] = e;
+          if (kPre) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-              const int sk = ee * 4 + k;
-              const V2 ws01 = s_ws[2 * sk], ws23 = s_ws[2 * sk + 1];
-              const V2 wd01 = s_wd[2 * sk], wd23 = s_wd[2 * sk + 1];
-              V2 v[4];
-              if (kPre) {
-                v[0] = val[q][4 * k + 0]; v[1] = val[q][4 * k + 1];
-                v[2] = val[q][4 * k + 2]; v[3] = val[q][4 * k + 3];
-              } else if (kGather == 1) {
-                v[0] = vn[0]; v[1] = vn[1]; v[2] = vn[2]; v[3] = vn[3];
-                if (k < 3) {
-                  const int sr = s_ex[sk + 1] ? sr_inv : sr_fwd;
-                  const int4 o = s_off[sk + 1];
-                  vn[0] = buf_load<V2>(rG, o.x + sr);
-                  vn[1] = buf_load<V2>(rG, o.y + sr);
-                  vn[2] = buf_load<V2>(rG, o.z + sr);
-                  vn[3] = buf_load<V2>(rG, o.w + sr);
-                }
-              } else {
-                const int sr = s_ex[sk] ? sr_inv : sr_fwd;
-                const int4 o = s_off[sk];
-                v[0] = buf_load<V2>(rG, o.x + sr);
-                v[1] = buf_load<V2>(rG, o.y + sr);
-                v[2] = buf_load<V2>(rG, o.z + sr);
-                v[3] = buf_load<V2>(rG, o.w + sr);
-              }
-              As[k] = fma(ws23.y, v[3].x, fma(ws23.x, v[2].x, fma(ws01.y, v[1].x, ws01.x * v[0].x)));
-              Ad[k] = fma(wd23.y, v[3].y, fma(wd23.x, v[2].y, fma(wd01.y, v[1].y, wd01.x * v[0].y)));
-            }
-            const double c = s_c[ee];
-            V rs, rd;
-            if (ch == 0) {
-              // integralKernelS (src/SU2/SU2FrgCore.cpp:199-227)
-              rs = V(-0.5) * As[0] * As[1] - V(0.5) * As[2] * As[3] + Ad[0] * As[1] + Ad[2] * As[3] + As[0] * Ad[1] +
-                   As[2] * Ad[3];
-              rd = V(0.1875) * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
-            } else if (ch == 2) {
-              // integralKernelU (:305-334)
-              rs = V(0.5) * As[0] * As[1] + V(0.5) * As[2] * As[3] + As[0] * Ad[1] + As[2] * Ad[3] + Ad[0] * As[1] +
-                   Ad[2] * As[3];
-              rd = V(0.1875) * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
-            } else {
-              // integralKernelT (:229-303) without the lattice bubble
-              const V *lc = s_loc + ee * 8;
-              const V Cbs = lc[0], Cbd = lc[1], ICas = lc[2], ICad = lc[3];
-              const V Cbs2 = lc[4], Cbd2 = lc[5], ICas2 = lc[6], ICad2 = lc[7];
-              rs = (V(0.25) * Cbs - Cbd) * As[0] + (V(0.25) * ICas - ICad) * As[1] + (V(0.25) * Cbs2 - Cbd2) * As[2] +
-                   (V(0.25) * ICas2 - ICad2) * As[3];
-              rd = -(Cbd + V(0.75) * Cbs) * Ad[0] - (ICad + V(0.75) * ICas) * Ad[1] -
-                   (Cbd2 + V(0.75) * Cbs2) * Ad[2] - (ICad2 + V(0.75) * ICas2) * Ad[3];
-              // lattice-bubble factors: P = c A2, Q = A3 per channel
-              s_P[(0 * nb + ee) * Lp + rid] = V(c) * As[2];
-              s_P[(1 * nb + ee) * Lp + rid] = V(c) * Ad[2];
-              s_Q[(0 * nb + ee) * Lp + rid] = As[3];
-              s_Q[(1 * nb + ee) * Lp + rid] = Ad[3];
-            }
-            if (MIXED) {
-              ksum_add(ks, float(c) * float(rs));
-              ksum_add(kd, float(c) * float(rd));
-            } else {
-              acc_s = fma(c, double(rs), acc_s);
-              acc_d = fma(c, double(rd), acc_d);
+              const int sk = ev[q] * 4 + k;
+              const int sr = (s_ex[sk] & 1) ? sr_inv : sr_fwd;
+              const int4 o = s_off[sk];
+              val[q][4 * k + 0] = buf_load<V2>(rG, o.x + sr);
+              val[q][4 * k + 1] = buf_load<V2>(rG, o.y + sr);
+              val[q][4 * k + 2] = buf_load<V2>(rG, o.z + sr);
+              val[q][4 * k + 3] = buf_load<V2>(rG, o.w + sr);
             }
           }
         }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+          if (!ok[q]) continue;
+          const int ee = ev[q];
+          V As[4], Ad[4];
+          V2 vn[4];
+          if (!kPre && kGather == 1) {
+            const int sr = (s_ex[ee * 4] & 1) ? sr_inv : sr_fwd;
+            const int4 o = s_off[ee * 4];
+            vn[0] = buf_load<V2>(rG, o.x + sr);
+            vn[1] = buf_load<V2>(rG, o.y + sr);
+            vn[2] = buf_load<V2>(rG, o.z + sr);
+            vn[3] = buf_load<V2>(rG, o.w + sr);
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int sk = ee * 4 + k;
+            const V2 ws01 = s_w[2 * sk], ws23 = s_w[2 * sk + 1];
+            const int exf = s_ex[sk];
+            // density weights carry the sign of the s <-> u exchanged storage
+            const V2 wd01 = mk2<V2>((exf & 2) ? -ws01.x : ws01.x, (exf & 4) ? -ws01.y : ws01.y);
+            const V2 wd23 = mk2<V2>((exf & 8) ? -ws23.x : ws23.x, (exf & 16) ? -ws23.y : ws23.y);
+            V2 v[4];
+            if (kPre) {
+              v[0] = val[q][4 * k + 0]; v[1] = val[q][4 * k + 1];
+              v[2] = val[q][4 * k + 2]; v[3] = val[q][4 * k + 3];
+            } else if (kGather == 1) {
+              v[0] = vn[0]; v[1] = vn[1]; v[2] = vn[2]; v[3] = vn[3];
+              if (k < 3) {
+                const int sr = (s_ex[sk + 1] & 1) ? sr_inv : sr_fwd;
+                const int4 o = s_off[sk + 1];
+                vn[0] = buf_load<V2>(rG, o.x + sr);
+                vn[1] = buf_load<V2>(rG, o.y + sr);
+                vn[2] = buf_load<V2>(rG, o.z + sr);
+                vn[3] = buf_load<V2>(rG, o.w + sr);
+              }
+            } else {
+              const int sr = (exf & 1) ? sr_inv : sr_fwd;
+              const int4 o = s_off[sk];
+              v[0] = buf_load<V2>(rG, o.x + sr);
+              v[1] = buf_load<V2>(rG, o.y + sr);
+              v[2] = buf_load<V2>(rG, o.z + sr);
+              v[3] = buf_load<V2>(rG, o.w + sr);
+            }
+            As[k] = fma(ws23.y, v[3].x, fma(ws23.x, v[2].x, fma(ws01.y, v[1].x, ws01.x * v[0].x)));
+            Ad[k] = fma(wd23.y, v[3].y, fma(wd23.x, v[2].y, fma(wd01.y, v[1].y, wd01.x * v[0].y)));
+          }
+          const double c = s_c[ee];
+          V rs, rd;
+          if (ch == 0) {
+            // integralKernelS (src/SU2/SU2FrgCore.cpp:199-227)
+            rs = V(-0.5) * As[0] * As[1] - V(0.5) * As[2] * As[3] + Ad[0] * As[1] + Ad[2] * As[3] + As[0] * Ad[1] +
+                 As[2] * Ad[3];
+            rd = V(0.1875) * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
+          } else if (ch == 2) {
+            // integralKernelU (:305-334)
+            rs = V(0.5) * As[0] * As[1] + V(0.5) * As[2] * As[3] + As[0] * Ad[1] + As[2] * Ad[3] + Ad[0] * As[1] +
+                 Ad[2] * As[3];
+            rd = V(0.1875) * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
+          } else {
+            // integralKernelT (:229-303) without the lattice bubble
+            const V *lc = s_loc + ee * 8;
+            const V Cbs = lc[0], Cbd = lc[1], ICas = lc[2], ICad = lc[3];
+            const V Cbs2 = lc[4], Cbd2 = lc[5], ICas2 = lc[6], ICad2 = lc[7];
+            rs = (V(0.25) * Cbs - Cbd) * As[0] + (V(0.25) * ICas - ICad) * As[1] + (V(0.25) * Cbs2 - Cbd2) * As[2] +
+                 (V(0.25) * ICas2 - ICad2) * As[3];
+            rd = -(Cbd + V(0.75) * Cbs) * Ad[0] - (ICad + V(0.75) * ICas) * Ad[1] -
+                 (Cbd2 + V(0.75) * Cbs2) * Ad[2] - (ICad2 + V(0.75) * ICas2) * Ad[3];
+            // lattice-bubble factors: P = c A2, Q = A3 per channel
+            s_P[(0 * nb + ee) * Lp + rid] = V(c) * As[2];
+            s_P[(1 * nb + ee) * Lp + rid] = V(c) * Ad[2];
+            s_Q[(0 * nb + ee) * Lp + rid] = As[3];
+            s_Q[(1 * nb + ee) * Lp + rid] = Ad[3];
+          }
+          if (MIXED) {
+            ksum_add(ks, float(c) * float(rs));
+            ksum_add(kd, float(c) * float(rd));
+          } else {
+            acc_s = fma(c, double(rs), acc_s);
+            acc_d = fma(c, double(rd), acc_d);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (jc >= 0 && ch == 1) {
+      // (C) C[ch][a][b] += sum_e P[ch][e][a] Q[ch][e][b], 2x2 register tiles
+      const int hp = Lp >> 1;
+      const int tiles = 2 * hp * hp;
+      for (int jt = tid; jt < tiles; jt += kThreads) {
+        const int cch = jt / (hp * hp);
+        const int r = jt - cch * hp * hp;
+        const int a = 2 * (r / hp), b = 2 * (r - (r / hp) * hp);
+        const V *P = s_P + cch * nb * Lp + a;
+        const V *Q = s_Q + cch * nb * Lp + b;
+        double *C0 = s_C + cch * Lp * Lp + a * Lp + b;
+        double *C1 = s_C + cch * Lp * Lp + (a + 1) * Lp + b;
+        if (MIXED) {
+          KSum c00, c01, c10, c11;
+          __builtin_memcpy(&c00, C0, 8);
+          __builtin_memcpy(&c01, C0 + 1, 8);
+          __builtin_memcpy(&c10, C1, 8);
+          __builtin_memcpy(&c11, C1 + 1, 8);
+          for (int e = 0; e < nvalid; ++e) {
+            const V2 p = *reinterpret_cast<const V2 *>(P + e * Lp);
+            const V2 q = *reinterpret_cast<const V2 *>(Q + e * Lp);
+            ksum_add(c00, float(p.x) * float(q.x));
+            ksum_add(c01, float(p.x) * float(q.y));
+            ksum_add(c10, float(p.y) * float(q.x));
+            ksum_add(c11, float(p.y) * float(q.y));
+          }
+          __builtin_memcpy(C0, &c00, 8);
+          __builtin_memcpy(C0 + 1, &c01, 8);
+          __builtin_memcpy(C1, &c10, 8);
+          __builtin_memcpy(C1 + 1, &c11, 8);
+        } else {
+          double2 c0 = *reinterpret_cast<double2 *>(C0), c1 = *reinterpret_cast<double2 *>(C1);
+          for (int e = 0; e < nvalid; ++e) {
+            const V2 p = *reinterpret_cast<const V2 *>(P + e * Lp);
+            const V2 q = *reinterpret_cast<const V2 *>(Q + e * Lp);
+            c0.x = fma(double(p.x), double(q.x), c0.x);
+            c0.y = fma(double(p.x), double(q.y), c0.y);
+            c1.x = fma(double(p.y), double(q.x), c1.x);
+            c1.y = fma(double(p.y), double(q.y), c1.y);
+          }
+          *reinterpret_cast<double2 *>(C0) = c0;
+          *reinterpret_cast<double2 *>(C1) = c1;
+        }
       }
       __syncthreads();
-      if (ch == 1) {
-        // (C) C[ch][a][b] += sum_e P[ch][e][a] Q[ch][e][b], 2x2 register tiles
-        const int hp = Lp >> 1;
-        const int tiles = 2 * hp * hp;
-        for (int j = tid; j < tiles; j += kThreads) {
-          const int cch = j / (hp * hp);
-          const int r = j - cch * hp * hp;
-          const int a = 2 * (r / hp), b = 2 * (r - (r / hp) * hp);
-          const V *P = s_P + cch * nb * Lp + a;
-          const V *Q = s_Q + cch * nb * Lp + b;
-          double *C0 = s_C + cch * Lp * Lp + a * Lp + b;
-          double *C1 = s_C + cch * Lp * Lp + (a + 1) * Lp + b;
-          if (MIXED) {
-            KSum c00, c01, c10, c11;
-            __builtin_memcpy(&c00, C0, 8);
-            __builtin_memcpy(&c01, C0 + 1, 8);
-            __builtin_memcpy(&c10, C1, 8);
-            __builtin_memcpy(&c11, C1 + 1, 8);
-            for (int e = 0; e < nvalid; ++e) {
-              const V2 p = *reinterpret_cast<const V2 *>(P + e * Lp);
-              const V2 q = *reinterpret_cast<const V2 *>(Q + e * Lp);
-              ksum_add(c00, float(p.x) * float(q.x));
-              ksum_add(c01, float(p.x) * float(q.y));
-              ksum_add(c10, float(p.y) * float(q.x));
-              ksum_add(c11, float(p.y) * float(q.y));
+      if (jc == cS + cT - 1) {
+        // RPA(rid) = sum over getOverlap(rid) of C[rid1][rid2] (src/SU2/SU2FrgCore.cpp:250-266)
+        const int wave = tid >> 6;
+        for (int r = wave; r < L; r += kThreads / 64) {
+          double ps = 0.0, pd = 0.0;
+          const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
+          for (int i = b + lane; i < e; i += 64) {
+            const int idx = F.ovl_idx[i];
+            if (MIXED) {
+              KSum a0, a1;
+              __builtin_memcpy(&a0, s_C + idx, 8);
+              __builtin_memcpy(&a1, s_C + Lp * Lp + idx, 8);
+              ps += double(a0.s) + double(a0.c);
+              pd += double(a1.s) + double(a1.c);
+            } else {
+              ps += s_C[idx];
+              pd += s_C[Lp * Lp + idx];
             }
-            __builtin_memcpy(C0, &c00, 8);
-            __builtin_memcpy(C0 + 1, &c01, 8);
-            __builtin_memcpy(C1, &c10, 8);
-            __builtin_memcpy(C1 + 1, &c11, 8);
-          } else {
-            double2 c0 = *reinterpret_cast<double2 *>(C0), c1 = *reinterpret_cast<double2 *>(C1);
-            for (int e = 0; e < nvalid; ++e) {
-              const V2 p = *reinterpret_cast<const V2 *>(P + e * Lp);
-              const V2 q = *reinterpret_cast<const V2 *>(Q + e * Lp);
-              c0.x = fma(double(p.x), double(q.x), c0.x);
-              c0.y = fma(double(p.x), double(q.y), c0.y);
-              c1.x = fma(double(p.y), double(q.x), c1.x);
-              c1.y = fma(double(p.y), double(q.y), c1.y);
-            }
-            *reinterpret_cast<double2 *>(C0) = c0;
-            *reinterpret_cast<double2 *>(C1) = c1;
           }
+          ps = wave_sum(ps);
+          pd = wave_sum(pd);
+          if (lane == 0) { s_rpa[2 * r] = 2.0 * F.spin * ps; s_rpa[2 * r + 1] = 8.0 * F.spin * pd; }
         }
         __syncthreads();
       }
-    }
-    if (ch == 1) {
-      // RPA(rid) = sum over getOverlap(rid) of C[rid1][rid2] (src/SU2/SU2FrgCore.cpp:250-266)
-      const int wave = tid >> 6, lane = tid & 63;
-      for (int r = wave; r < L; r += kThreads / 64) {
-        double ps = 0.0, pd = 0.0;
-        const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
-        for (int i = b + lane; i < e; i += 64) {
-          const int idx = F.ovl_idx[i];
-          if (MIXED) {
-            KSum a0, a1;
-            __builtin_memcpy(&a0, s_C + idx, 8);
-            __builtin_memcpy(&a1, s_C + Lp * Lp + idx, 8);
-            ps += double(a0.s) + double(a0.c);
-            pd += double(a1.s) + double(a1.c);
-          } else {
-            ps += s_C[idx];
-            pd += s_C[Lp * Lp + idx];
-          }
-        }
-        ps = wave_sum(ps);
-        pd = wave_sum(pd);
-        if (lane == 0) { s_rpa[2 * r] = 2.0 * F.spin * ps; s_rpa[2 * r + 1] = 8.0 * F.spin * pd; }
-      }
-      __syncthreads();
     }
   }
 
@@ -871,7 +912,10 @@ struct ProfScope {
 };
 
 int check_problem(sp_ctx *ctx, const sp_su2_problem *p) {
-  if (!p || p->nw < 2 || p->nw > 4096 || !p->mesh) return fail(ctx, SP_EARG, "invalid frequency mesh");
+  if (!p || p->nw < 2 || !p->mesh) return fail(ctx, SP_EARG, "invalid frequency mesh");
+  // the dGamma kernel stages a channel's 2 N_w + 8 evaluation points with two
+  // loads per thread
+  if (2 * p->nw + 8 > 2 * kThreads) return fail(ctx, SP_EARG, "at most 252 positive frequencies are supported");
   for (int i = 0; i < p->nw; ++i)
     if (!(p->mesh[i] > 0.0f) || (i > 0 && !(p->mesh[i] > p->mesh[i - 1])))
       return fail(ctx, SP_EARG, "mesh must be positive and strictly ascending");
@@ -898,12 +942,14 @@ int check_problem(sp_ctx *ctx, const sp_su2_problem *p) {
   return SP_OK;
 }
 
+// worker slots (waves 0..2) and points per chunk; the producer wave holds one
+// lane per (point, pair), so nb <= 32
 void launch_shape(int nw, int L, int vs, int &slots, int &nb, size_t &lds) {
-  slots = kThreads / L;
-  int per = kMaxBatch / slots;
+  slots = (kThreads - 64) / L;
+  int per = (vs == 4 ? kMaxBatchMixed : kMaxBatch) / slots;
   if (per < 1) per = 1;
   nb = slots * per;
-  if (nb > kMaxBatch) nb = kMaxBatch;
+  if (nb > 32) nb = 32;
   lds = (size_t)lds_layout(nw, L, nb, slots, vs).total;
 }
 
