@@ -346,12 +346,6 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
   for (int i = tid; i < 2 * L; i += kThreads) s_rpa[i] = 0.0;
   __syncthreads();
 
-  const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
-  const double w1p = 0.5 * (s + t + u);
-  const double w1 = 0.5 * (s - t + u);
-  const double w2p = 0.5 * (s - t - u);
-  const double w2 = 0.5 * (s + t - u);
-
   const int lane = tid & 63;
   const bool producer = tid >= kThreads - 64;
   const int slot = tid / L;
@@ -388,6 +382,13 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
         const int n = ch == 0 ? nS : (ch == 1 ? nT : nU);
         const int nvalid = n - e0 < nb ? n - e0 : nb;
         if (lane < 2 * nvalid) {
+          // transfer frequencies (SU2FrgCore.cpp:193-197), recomputed here so
+          // that they occupy no registers in the worker code
+          const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
+          const double w1p = 0.5 * (s + t + u);
+          const double w1 = 0.5 * (s - t + u);
+          const double w2p = 0.5 * (s - t - u);
+          const double w2 = 0.5 * (s + t - u);
           double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
           int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
           V2 *s_w = reinterpret_cast<V2 *>(lds_raw + lay.w + bf * lay.dbuf);
