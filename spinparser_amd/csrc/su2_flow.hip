@@ -239,7 +239,7 @@ __global__ __launch_bounds__(64) void k_dsigma(const FlowDev *__restrict__ fl) {
 // off, w, ex, loc) are double-buffered: the producer wave fills one buffer
 // while the worker waves read the other.
 struct LdsLayout {
-  int mesh, c, off, w, ex, loc, dbuf, P, Q, C, rpa, red, site, inv, total;  // buffer b at + b * dbuf
+  int mesh, c, off, w, wd, ex, loc, dbuf, P, Q, C, rpa, red, site, inv, total;  // buffer b at + b * dbuf
 };
 
 __host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots, int vs) {
@@ -252,6 +252,7 @@ __host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots
   l.c = take(nb * 8);
   l.off = take(nb * 4 * 16);
   l.w = take(nb * 4 * 4 * vs);
+  l.wd = take(nb * 4 * 4 * vs);
   l.ex = take(nb * 4 * 4);
   l.loc = take(nb * 4 * 2 * vs);
   l.dbuf = off - d0;
@@ -392,6 +393,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
           double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
           int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
           V2 *s_w = reinterpret_cast<V2 *>(lds_raw + lay.w + bf * lay.dbuf);
+          V2 *s_wd = reinterpret_cast<V2 *>(lds_raw + lay.wd + bf * lay.dbuf);
           int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex + bf * lay.dbuf);
           V *s_loc = reinterpret_cast<V *>(lds_raw + lay.loc + bf * lay.dbuf);
           const int e = lane >> 1, pr = lane & 1;
@@ -449,7 +451,9 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
               s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
               s_w[2 * sk] = mk2<V2>(V(d.ws[0]), V(d.ws[1]));
               s_w[2 * sk + 1] = mk2<V2>(V(d.ws[2]), V(d.ws[3]));
-              s_ex[sk] = (((bb.a < 0) != (bb.c < 0)) ? 1 : 0) | (d.neg << 1);
+              s_wd[2 * sk] = mk2<V2>(V(d.wd[0]), V(d.wd[1]));
+              s_wd[2 * sk + 1] = mk2<V2>(V(d.wd[2]), V(d.wd[3]));
+              s_ex[sk] = ((bb.a < 0) != (bb.c < 0)) ? 1 : 0;
             } else {
               // SU2VertexTwoParticle::getValueLocal (:347-360): site 0, no exchange
               V vs = V(0), vd = V(0);
@@ -459,8 +463,10 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
                 vs = fma(V(d.ws[i]), v.x, vs);
                 vd = fma(V(d.wd[i]), v.y, vd);
               }
-              s_loc[(e * 4 + (bb.k - 4)) * 2 + 0] = vs;
-              s_loc[(e * 4 + (bb.k - 4)) * 2 + 1] = vd;
+              // the factors the t-channel algebra applies to this buffer's
+              // partner bundle: spin 1/4 vs - vd, density -(3/4 vs + vd)
+              s_loc[(e * 4 + (bb.k - 4)) * 2 + 0] = fma(V(0.25), vs, -vd);
+              s_loc[(e * 4 + (bb.k - 4)) * 2 + 1] = -fma(V(0.75), vs, vd);
             }
           }
           if (pr == 0) s_c[e] = (ch == 2 ? -1.0 : 1.0) * F.Wc[(size_t)xo * F.emax + e0 + e];
@@ -472,6 +478,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
       const double *s_c = reinterpret_cast<const double *>(lds_raw + lay.c + cur * lay.dbuf);
       const int4 *s_off = reinterpret_cast<const int4 *>(lds_raw + lay.off + cur * lay.dbuf);
       const V2 *s_w = reinterpret_cast<const V2 *>(lds_raw + lay.w + cur * lay.dbuf);
+      const V2 *s_wd = reinterpret_cast<const V2 *>(lds_raw + lay.wd + cur * lay.dbuf);
       const int *s_ex = reinterpret_cast<const int *>(lds_raw + lay.ex + cur * lay.dbuf);
       const V *s_loc = reinterpret_cast<const V *>(lds_raw + lay.loc + cur * lay.dbuf);
       constexpr int U = MIXED ? SPFRG_UNROLL_MIXED : SPFRG_UNROLL_FP64;
@@ -519,10 +526,9 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
           for (int k = 0; k < 4; ++k) {
             const int sk = ee * 4 + k;
             const V2 ws01 = s_w[2 * sk], ws23 = s_w[2 * sk + 1];
-            const int exf = s_ex[sk];
             // density weights carry the sign of the s <-> u exchanged storage
-            const V2 wd01 = mk2<V2>((exf & 2) ? -ws01.x : ws01.x, (exf & 4) ? -ws01.y : ws01.y);
-            const V2 wd23 = mk2<V2>((exf & 8) ? -ws23.x : ws23.x, (exf & 16) ? -ws23.y : ws23.y);
+            const V2 wd01 = s_wd[2 * sk], wd23 = s_wd[2 * sk + 1];
+            const int exf = s_ex[sk];
             V2 v[4];
             if (kPre) {
               v[0] = val[q][4 * k + 0]; v[1] = val[q][4 * k + 1];
@@ -550,25 +556,21 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
           }
           const double c = s_c[ee];
           V rs, rd;
-          if (ch == 0) {
-            // integralKernelS (src/SU2/SU2FrgCore.cpp:199-227)
-            rs = V(-0.5) * As[0] * As[1] - V(0.5) * As[2] * As[3] + Ad[0] * As[1] + Ad[2] * As[3] + As[0] * Ad[1] +
-                 As[2] * Ad[3];
-            rd = V(0.1875) * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
-          } else if (ch == 2) {
-            // integralKernelU (:305-334)
-            rs = V(0.5) * As[0] * As[1] + V(0.5) * As[2] * As[3] + As[0] * Ad[1] + As[2] * Ad[3] + Ad[0] * As[1] +
-                 Ad[2] * As[3];
-            rd = V(0.1875) * (As[0] * As[1] + As[2] * As[3]) + Ad[0] * Ad[1] + Ad[2] * Ad[3];
+          if (ch != 1) {
+            // integralKernelS (src/SU2/SU2FrgCore.cpp:199-227), integralKernelU
+            // (:305-334): with h = As0 As1 + As2 As3,
+            // S: rs = -h/2 + Ad0 As1 + Ad2 As3 + As0 Ad1 + As2 Ad3, U: rs = h/2 + (same)
+            // rd = 3/16 h + Ad0 Ad1 + Ad2 Ad3 for both
+            const V h = fma(As[0], As[1], As[2] * As[3]);
+            const V x = fma(Ad[0], As[1], fma(Ad[2], As[3], fma(As[0], Ad[1], As[2] * Ad[3])));
+            rs = fma(ch == 0 ? V(-0.5) : V(0.5), h, x);
+            rd = fma(V(0.1875), h, fma(Ad[0], Ad[1], Ad[2] * Ad[3]));
           } else {
-            // integralKernelT (:229-303) without the lattice bubble
+            // integralKernelT (:229-303) without the lattice bubble: the
+            // site-local vertices enter as the factors built by the producer
             const V *lc = s_loc + ee * 8;
-            const V Cbs = lc[0], Cbd = lc[1], ICas = lc[2], ICad = lc[3];
-            const V Cbs2 = lc[4], Cbd2 = lc[5], ICas2 = lc[6], ICad2 = lc[7];
-            rs = (V(0.25) * Cbs - Cbd) * As[0] + (V(0.25) * ICas - ICad) * As[1] + (V(0.25) * Cbs2 - Cbd2) * As[2] +
-                 (V(0.25) * ICas2 - ICad2) * As[3];
-            rd = -(Cbd + V(0.75) * Cbs) * Ad[0] - (ICad + V(0.75) * ICas) * Ad[1] -
-                 (Cbd2 + V(0.75) * Cbs2) * Ad[2] - (ICad2 + V(0.75) * ICas2) * Ad[3];
+            rs = fma(lc[0], As[0], fma(lc[2], As[1], fma(lc[4], As[2], lc[6] * As[3])));
+            rd = fma(lc[1], Ad[0], fma(lc[3], Ad[1], fma(lc[5], Ad[2], lc[7] * Ad[3])));
             // lattice-bubble factors: P = c A2, Q = A3 per channel
             s_P[(0 * nb + ee) * Lp + rid] = V(c) * As[2];
             s_P[(1 * nb + ee) * Lp + rid] = V(c) * Ad[2];
