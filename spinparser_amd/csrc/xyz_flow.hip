@@ -20,18 +20,20 @@
 //   Sigma, dSigma  FP64 [N_w]
 //   weight tables  per exact index x: evaluation points and weights at the cutoff
 //
-// dGamma kernel (k_xyz_dgamma): one workgroup of 256 threads per frequency
-// triple (s, t, u), XCD-chunked like the SU(2) kernel; threads are (slot, rid)
-// pairs.  Per chunk of evaluation points:
-//   A  one thread per (point, pair of access buffers): the two bracket searches
-//      the pair shares, descriptors (support byte offsets, weights, density
-//      signs, exchange) -> LDS; the T channel's four site-local vertices;
-//   B  per (slot, rid): 4 buffers x 4 supports x 2 loads, the site's spin
-//      permutation, the S / U ladder or the chalice terms; in the T channel
-//      the bundles of the first buffer pair go to LDS;
-//   C  (T channel) the lattice bubble as rank updates
+// dGamma kernel (k_xyz_dgamma): one workgroup of 384 threads per frequency
+// triple (s, t, u), XCD-chunked like the SU(2) kernel.  Waves 0..4 are
+// workers, (slot, rid) pairs; wave 5 is the producer.  The evaluation points
+// of the three channels run as one sequence of chunks:
+//   A  (producer, one chunk ahead, double-buffered LDS) one lane per (point,
+//      pair of access buffers): the two bracket searches the pair shares,
+//      descriptors (support byte offsets, weights, density signs, exchange);
+//      the T channel's four site-local vertices, stored as chalice factors;
+//   B  (workers) per (slot, rid): 4 buffers x 4 supports x 2 loads, the site's
+//      spin permutation, the S / U ladder or the chalice terms; in the T
+//      channel the bundles of the first buffer pair go to LDS;
+//   C  (T channel, all waves) the lattice bubble as rank updates
 //      M[a][b] += sum_e c_e A0_e[a] (x) A1_e[b] for every spin-component pair
-//      (a, b) the overlap transformations use (2x2 register tiles over LDS),
+//      (a, b) the overlap transformations use (4x4 register tiles over LDS),
 //      contracted with the overlap lists once per item.
 // When the bubble matrices do not fit the LDS of a workgroup, k_xyz_dgamma_direct
 // evaluates the bubble per point from the overlap lists instead.
@@ -54,6 +56,20 @@ using namespace frg;
 
 constexpr int kXyzThreads = 256;
 constexpr int kXyzMaxPoints = 16;  // evaluation points per chunk of the direct kernel
+// launch shape of the producer/worker dGamma kernel (compile-time tunables;
+// the defaults are the tuned ones): threads per workgroup, evaluation points
+// per worker slot and chunk, workgroups per CU the register budget is set for
+#ifndef SPFRG_XYZ_THREADS
+#define SPFRG_XYZ_THREADS 384
+#endif
+#ifndef SPFRG_XYZ_PER
+#define SPFRG_XYZ_PER 1
+#endif
+#ifndef SPFRG_XYZ_MIN_BLOCKS
+#define SPFRG_XYZ_MIN_BLOCKS 2
+#endif
+constexpr int kXyzPW = SPFRG_XYZ_THREADS;
+constexpr int kXyzPer = SPFRG_XYZ_PER;
 constexpr long long kXyzChunk = 16;
 constexpr int kX = 0, kY = 1, kZ = 2, kD = 3;
 constexpr int kMaxPairs = 10;  // 9 spin-component pairs + density
@@ -195,22 +211,26 @@ __device__ __forceinline__ double pick3(double u0, double u1, double u2, int i) 
 }
 
 // LDS carve-up of k_xyz_dgamma (bytes), identical on host and device.  Lp = L
-// rounded up to even (2x2 tiles of the bubble matrices).
+// rounded up to a multiple of 4 (4x4 register tiles of the bubble matrices).  The per-chunk
+// descriptors (c, off, ws, ex, loc) are double-buffered: the producer wave
+// fills one buffer while the worker waves read the other.
 struct XyzLds {
-  int mesh, c, off, ws, wd, ex, loc, P, Q, M, rpa, red, total;
+  int mesh, c, off, ws, ex, loc, dbuf, P, Q, M, rpa, red, total;  // buffer b at + b * dbuf
 };
 __host__ __device__ inline XyzLds xyz_lds(int nw, int L, int nb, int slots, int npairs) {
   XyzLds l;
-  const int Lp = L + (L & 1);
+  const int Lp = (L + 3) & ~3;
   int o = 0;
   auto take = [&](int bytes) { int r = o; o += (bytes + 15) & ~15; return r; };
   l.mesh = take(nw * 8);
+  const int d0 = o;
   l.c = take(nb * 8);
   l.off = take(nb * 4 * 16);
   l.ws = take(nb * 4 * 4 * 8);
-  l.wd = take(nb * 4 * 4 * 8);
   l.ex = take(nb * 4 * 4);
-  l.loc = take(nb * 4 * 4 * 8);
+  l.loc = take(nb * 16 * 8);
+  l.dbuf = o - d0;
+  o += l.dbuf;  // second descriptor buffer
   const int pq = 4 * nb * Lp * 8;
   const int red = slots * L * 4 * 8;
   l.P = take(pq > red ? pq : red);  // the final slot reduction reuses P
@@ -222,7 +242,13 @@ __host__ __device__ inline XyzLds xyz_lds(int nw, int L, int nb, int slots, int 
   return l;
 }
 
-__global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
+// dGamma kernel.  Waves 0 .. kXyzPW/64 - 2 are workers, (slot, rid) pairs with
+// slots = floor((kXyzPW - 64) / L); the last wave is the producer.  The
+// evaluation points of the three channels form one sequence of chunks of nb
+// points; while the workers gather and evaluate chunk j, the producer builds
+// the descriptors of chunk j + 1 (bracket searches, weights, the t channel's
+// site-local vertices) into the other LDS buffer.
+__global__ __launch_bounds__(kXyzPW, SPFRG_XYZ_MIN_BLOCKS) void k_xyz_dgamma(XyzDev F) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const long long total = F.nf;
   const long long j0 = blockIdx.x >> 3;
@@ -230,15 +256,9 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
   if (item >= total) return;
 
   const int nw = F.nw, L = F.L, nb = F.nb, slots = F.slots, npairs = F.npairs;
-  const int Lp = L + (L & 1);
+  const int Lp = (L + 3) & ~3;
   const XyzLds lay = xyz_lds(nw, L, nb, slots, npairs);
   double *s_mesh = reinterpret_cast<double *>(lds_raw + lay.mesh);
-  double *s_c = reinterpret_cast<double *>(lds_raw + lay.c);
-  int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off);
-  double *s_ws = reinterpret_cast<double *>(lds_raw + lay.ws);
-  double *s_wd = reinterpret_cast<double *>(lds_raw + lay.wd);
-  int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex);
-  double *s_loc = reinterpret_cast<double *>(lds_raw + lay.loc);
   double *s_P = reinterpret_cast<double *>(lds_raw + lay.P);
   double *s_Q = reinterpret_cast<double *>(lds_raw + lay.Q);
   double *s_M = reinterpret_cast<double *>(lds_raw + lay.M);
@@ -257,20 +277,17 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
   while (so * (so + 1) / 2 > su) --so;
   const int uo = su - so * (so + 1) / 2;
 
-  for (int i = tid; i < nw; i += kXyzThreads) s_mesh[i] = F.mesh[i];
-  for (int i = tid; i < npairs * Lp * Lp; i += kXyzThreads) s_M[i] = 0.0;
-  for (int i = tid; i < 4 * nb * Lp; i += kXyzThreads) { s_P[i] = 0.0; s_Q[i] = 0.0; }
+  for (int i = tid; i < nw; i += kXyzPW) s_mesh[i] = F.mesh[i];
+  for (int i = tid; i < npairs * Lp * Lp; i += kXyzPW) s_M[i] = 0.0;
+  for (int i = tid; i < 4 * nb * Lp; i += kXyzPW) { s_P[i] = 0.0; s_Q[i] = 0.0; }
+  for (int i = tid; i < 4 * L; i += kXyzPW) s_rpa[i] = 0.0;
   __syncthreads();
 
-  const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
-  const double w1p = 0.5 * (s + t + u);
-  const double w1 = 0.5 * (s - t + u);
-  const double w2p = 0.5 * (s - t - u);
-  const double w2 = 0.5 * (s + t - u);
-
+  const int lane = tid & 63;
+  const bool producer = tid >= kXyzPW - 64;
   const int slot = tid / L;
   const int rid = tid - slot * L;
-  const bool active = slot < slots;
+  const bool active = slot < slots;  // slots * L <= kXyzPW - 64: workers only
   // byte offsets of this thread's site (plain / exchanged) and the packed spin
   // permutations (2 bits per component) of getSites / getInvertedSites
   int sr_fwd = 0, sr_inv = 0, pf = 0, pi = 0;
@@ -282,203 +299,251 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
   }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
 
-  for (int ch = 0; ch < 3; ++ch) {
-    const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
-    const int n = F.Wn[xo];
-    const double *Wwp = F.Wwp + (size_t)xo * F.emax;
-    const double *Wc = F.Wc + (size_t)xo * F.emax;
-    for (int e0 = 0; e0 < n; e0 += nb) {
-      const int nvalid = n - e0 < nb ? n - e0 : nb;
-      // (A) one thread per (point, pair): the two bracket searches shared by
-      // the 2 (s, u channels) or 4 (t channel) access buffers of the pair
-      // (XYZFrgCore.cpp:226-230, 281-287, 325-331, 423-427)
-      if (tid < 2 * nvalid) {
-        const int e = tid >> 1, pr = tid & 1;
-        const double wp = Wwp[e0 + e];
-        double x0, x1;
-        PairBuf pb[4];
-        int nbufs = 2;
-        if (ch == 0) {
-          if (pr == 0) {
-            x0 = fabs(w1 + wp); x1 = fabs(w2 + wp);
-            pb[0] = {0, s, -w2 - wp, false, 0};
-            pb[1] = {2, s, w1 + wp, true, 0};
-          } else {
-            x0 = fabs(w1p + wp); x1 = fabs(w2p + wp);
-            pb[0] = {1, s, -w2p - wp, false, 0};
-            pb[1] = {3, s, w1p + wp, true, 0};
-          }
-        } else if (ch == 1) {
-          nbufs = 4;
-          if (pr == 0) {
-            x0 = fabs(w1 - wp); x1 = fabs(w1p + wp);
-            pb[0] = {0, w1 - wp, w1p + wp, false, 1};
-            pb[1] = {2, w1p + wp, w1 - wp, true, 1};
-            pb[2] = {5, w1 - wp, -t, false, 2};
-            pb[3] = {7, w1p + wp, -t, true, 2};
-          } else {
-            x0 = fabs(w2p - wp); x1 = fabs(w2 + wp);
-            pb[0] = {1, w2p - wp, -w2 - wp, false, 1};
-            pb[1] = {3, w2 + wp, wp - w2p, true, 1};
-            pb[2] = {4, w2p - wp, t, false, 2};
-            pb[3] = {6, w2 + wp, t, true, 2};
-          }
-        } else {
-          if (pr == 0) {
-            x0 = fabs(w1 + wp); x1 = fabs(w2p - wp);
-            pb[0] = {0, w1 + wp, u, false, 2};
-            pb[1] = {2, w2p - wp, u, true, 2};
-          } else {
-            x0 = fabs(w1p + wp); x1 = fabs(w2 - wp);
-            pb[0] = {1, w1p + wp, u, false, 2};
-            pb[1] = {3, w2 - wp, u, true, 2};
-          }
-        }
-        const Interp i0 = interp_of(s_mesh, nw, x0);
-        const Interp i1 = interp_of(s_mesh, nw, x1);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (q >= nbufs) break;
-          const PairBuf &b = pb[q];
-          const Desc4 d = assemble(b.chan, xo, b.swap ? i1 : i0, b.swap ? i0 : i1, nw, rowbytes);
-          if (b.k < 4) {
-            const int sk = e * 4 + b.k;
-            s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) { s_ws[sk * 4 + r] = d.ws[r]; s_wd[sk * 4 + r] = d.wd[r]; }
-            s_ex[sk] = ((b.a < 0) != (b.c < 0)) ? 1 : 0;
-          } else {
-            // getValueLocal (XYZVertexTwoParticle.hpp:355-376): site 0, no exchange
-            double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const double2 v01 = buf_load<double2>(rG, d.off[r]);
-              const double2 v23 = buf_load<double2>(rG, d.off[r] + 16);
-              l0 = fma(d.ws[r], v01.x, l0);
-              l1 = fma(d.ws[r], v01.y, l1);
-              l2 = fma(d.ws[r], v23.x, l2);
-              l3 = fma(d.wd[r], v23.y, l3);
+  // chunk schedule: chunk j covers points [e0, e0 + nb) of channel ch
+  const int nS = F.Wn[so], nT = F.Wn[to], nU = F.Wn[uo];
+  const int cS = (nS + nb - 1) / nb, cT = (nT + nb - 1) / nb, cU = (nU + nb - 1) / nb;
+  const int nchunks = cS + cT + cU;
+
+  // iteration jc: the producer builds chunk jc + 1 while the workers evaluate
+  // chunk jc (jc = -1: the first chunk's descriptors only)
+  for (int jc = -1; jc < nchunks; ++jc) {
+    const int cur = jc & 1;
+    const int ch = jc < cS ? 0 : (jc < cS + cT ? 1 : 2);
+    const int e0 = (jc - (ch == 0 ? 0 : (ch == 1 ? cS : cS + cT))) * nb;
+    const int n = ch == 0 ? nS : (ch == 1 ? nT : nU);
+    const int nvalid = n - e0 < nb ? n - e0 : nb;
+    if (producer) {
+      const int jj = jc + 1, bf = jj & 1;
+      if (jj < nchunks) {
+        const int ch = jj < cS ? 0 : (jj < cS + cT ? 1 : 2);
+        const int e0 = (jj - (ch == 0 ? 0 : (ch == 1 ? cS : cS + cT))) * nb;
+        const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
+        const int n = ch == 0 ? nS : (ch == 1 ? nT : nU);
+        const int nvalid = n - e0 < nb ? n - e0 : nb;
+        // (A) one lane per (point, pair): the two bracket searches shared by
+        // the 2 (s, u channels) or 4 (t channel) access buffers of the pair
+        // (XYZFrgCore.cpp:226-230, 281-287, 325-331, 423-427)
+        if (lane < 2 * nvalid) {
+          const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
+          const double w1p = 0.5 * (s + t + u);
+          const double w1 = 0.5 * (s - t + u);
+          const double w2p = 0.5 * (s - t - u);
+          const double w2 = 0.5 * (s + t - u);
+          double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
+          int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
+          double *s_ws = reinterpret_cast<double *>(lds_raw + lay.ws + bf * lay.dbuf);
+          int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex + bf * lay.dbuf);
+          double *s_loc = reinterpret_cast<double *>(lds_raw + lay.loc + bf * lay.dbuf);
+          const int e = lane >> 1, pr = lane & 1;
+          const double wp = F.Wwp[(size_t)xo * F.emax + e0 + e];
+          const double c = F.Wc[(size_t)xo * F.emax + e0 + e];
+          double x0, x1;
+          PairBuf pb[4];
+          int nbufs = 2;
+          if (ch == 0) {
+            if (pr == 0) {
+              x0 = fabs(w1 + wp); x1 = fabs(w2 + wp);
+              pb[0] = {0, s, -w2 - wp, false, 0};
+              pb[1] = {2, s, w1 + wp, true, 0};
+            } else {
+              x0 = fabs(w1p + wp); x1 = fabs(w2p + wp);
+              pb[0] = {1, s, -w2p - wp, false, 0};
+              pb[1] = {3, s, w1p + wp, true, 0};
             }
-            double *lo = s_loc + (e * 4 + (b.k - 4)) * 4;
-            lo[0] = l0; lo[1] = l1; lo[2] = l2; lo[3] = l3;
-          }
-        }
-        if (pr == 0) s_c[e] = Wc[e0 + e];
-      }
-      __syncthreads();
-      // (B) gathers and channel algebra per (slot, rid)
-      if (active) {
-        for (int e = slot; e < nvalid; e += slots) {
-          double A[4][4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int sk = e * 4 + k;
-            const bool ex = s_ex[sk] != 0;
-            const int sr = ex ? sr_inv : sr_fwd;
-            const int pm = ex ? pi : pf;
-            const int4 o = s_off[sk];
-            const int oo[4] = {o.x, o.y, o.z, o.w};
-            double u0 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const double2 v01 = buf_load<double2>(rG, oo[r] + sr);
-              const double2 v23 = buf_load<double2>(rG, oo[r] + sr + 16);
-              const double ws = s_ws[sk * 4 + r], wd = s_wd[sk * 4 + r];
-              u0 = fma(ws, v01.x, u0);
-              u1 = fma(ws, v01.y, u1);
-              u2 = fma(ws, v23.x, u2);
-              u3 = fma(wd, v23.y, u3);
+          } else if (ch == 1) {
+            nbufs = 4;
+            if (pr == 0) {
+              x0 = fabs(w1 - wp); x1 = fabs(w1p + wp);
+              pb[0] = {0, w1 - wp, w1p + wp, false, 1};
+              pb[1] = {2, w1p + wp, w1 - wp, true, 1};
+              pb[2] = {5, w1 - wp, -t, false, 2};
+              pb[3] = {7, w1p + wp, -t, true, 2};
+            } else {
+              x0 = fabs(w2p - wp); x1 = fabs(w2 + wp);
+              pb[0] = {1, w2p - wp, -w2 - wp, false, 1};
+              pb[1] = {3, w2 + wp, wp - w2p, true, 1};
+              pb[2] = {4, w2p - wp, t, false, 2};
+              pb[3] = {6, w2 + wp, t, true, 2};
             }
-            // bundle component c of the site is the stored component perm[c]
-            // (getValueSuperbundle, XYZVertexTwoParticle.hpp:385-407)
-            A[k][kX] = pick3(u0, u1, u2, pm & 3);
-            A[k][kY] = pick3(u0, u1, u2, (pm >> 2) & 3);
-            A[k][kZ] = pick3(u0, u1, u2, (pm >> 4) & 3);
-            A[k][kD] = u3;
+          } else {
+            if (pr == 0) {
+              x0 = fabs(w1 + wp); x1 = fabs(w2p - wp);
+              pb[0] = {0, w1 + wp, u, false, 2};
+              pb[1] = {2, w2p - wp, u, true, 2};
+            } else {
+              x0 = fabs(w1p + wp); x1 = fabs(w2 - wp);
+              pb[0] = {1, w1p + wp, u, false, 2};
+              pb[1] = {3, w2 - wp, u, true, 2};
+            }
           }
-          const double c = s_c[e];
-          double a0[4], b1[4], a2[4], b3[4];
+          Interp i0, i1;
+          interp_pair(s_mesh, nw, x0, x1, i0, i1);
+          // fully unrolled with a guard so that pb[] stays in registers
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            a0[q] = c * A[0][q]; a2[q] = c * A[2][q];
-            b1[q] = A[1][q]; b3[q] = A[3][q];
-          }
-          if (ch != 1) {
-            const double sx = ch == 0 ? 1.0 : -1.0;
-            ladder(acc, a0, b1, sx);
-            ladder(acc, a2, b3, sx);
-          } else {
-            // chalice with Cb / Cb2, inverse chalice with ICa / ICa2 (locals 4..7)
-            const double *lc = s_loc + e * 16;
-            double cb1[4], cb3[4];
+            if (q >= nbufs) break;
+            const PairBuf &bb = pb[q];
+            const Desc4 d = assemble(bb.chan, xo, bb.swap ? i1 : i0, bb.swap ? i0 : i1, nw, rowbytes);
+            if (bb.k < 4) {
+              const int sk = e * 4 + bb.k;
+              s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) { cb1[q] = c * b1[q]; cb3[q] = c * b3[q]; }
-            chalices(acc, a0, cb1, lc, lc + 4);
-            chalices(acc, a2, cb3, lc + 8, lc + 12);
-            // lattice-bubble factors of the first pair: P = c A0, Q = A1
+              for (int r = 0; r < 4; ++r) s_ws[sk * 4 + r] = d.ws[r];
+              // bit 0: site exchange; bits 1..4: density weight of support r negated
+              s_ex[sk] = (((bb.a < 0) != (bb.c < 0)) ? 1 : 0) | (d.neg << 1);
+            } else {
+              // getValueLocal (XYZVertexTwoParticle.hpp:355-376): site 0, no exchange
+              double l[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              s_P[(q * nb + e) * Lp + rid] = a0[q];
-              s_Q[(q * nb + e) * Lp + rid] = b1[q];
+              for (int r = 0; r < 4; ++r) {
+                const double2 v01 = buf_load<double2>(rG, d.off[r]);
+                const double2 v23 = buf_load<double2>(rG, d.off[r] + 16);
+                l[kX] = fma(d.ws[r], v01.x, l[kX]);
+                l[kY] = fma(d.ws[r], v01.y, l[kY]);
+                l[kZ] = fma(d.ws[r], v23.x, l[kZ]);
+                l[kD] = fma(d.wd[r], v23.y, l[kD]);
+              }
+              // chalice / inverse chalice factors of this local vertex, scaled
+              // by the point's weight (XYZFrgCore.cpp:333-416): spin output k
+              // gets the two other spin components with +, the component
+              // itself and the density with -; the density output -sum
+              double *fo = s_loc + e * 16 + (bb.k - 4) * 4;
+#pragma unroll
+              for (int k = 0; k < 3; ++k) fo[k] = c * chal(l, k);
+              fo[kD] = -(c * ((l[kD] + l[kX]) + (l[kY] + l[kZ])));
             }
+          }
+          if (pr == 0) s_c[e] = c;
+        }
+      }
+    } else if (active && jc >= 0) {
+      // (B) gathers and channel algebra per (slot, rid)
+      const double *s_c = reinterpret_cast<const double *>(lds_raw + lay.c + cur * lay.dbuf);
+      const int4 *s_off = reinterpret_cast<const int4 *>(lds_raw + lay.off + cur * lay.dbuf);
+      const double *s_ws = reinterpret_cast<const double *>(lds_raw + lay.ws + cur * lay.dbuf);
+      const int *s_ex = reinterpret_cast<const int *>(lds_raw + lay.ex + cur * lay.dbuf);
+      const double *s_loc = reinterpret_cast<const double *>(lds_raw + lay.loc + cur * lay.dbuf);
+      for (int e = slot; e < nvalid; e += slots) {
+        double A[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int sk = e * 4 + k;
+          const int exw = s_ex[sk];
+          const bool ex = (exw & 1) != 0;
+          const int sr = ex ? sr_inv : sr_fwd;
+          const int pm = ex ? pi : pf;
+          const int4 o = s_off[sk];
+          const int oo[4] = {o.x, o.y, o.z, o.w};
+          double u0 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double2 v01 = buf_load<double2>(rG, oo[r] + sr);
+            const double2 v23 = buf_load<double2>(rG, oo[r] + sr + 16);
+            const double ws = s_ws[sk * 4 + r];
+            const double wd = (exw >> (1 + r)) & 1 ? -ws : ws;
+            u0 = fma(ws, v01.x, u0);
+            u1 = fma(ws, v01.y, u1);
+            u2 = fma(ws, v23.x, u2);
+            u3 = fma(wd, v23.y, u3);
+          }
+          // bundle component c of the site is the stored component perm[c]
+          // (getValueSuperbundle, XYZVertexTwoParticle.hpp:385-407)
+          A[k][kX] = pick3(u0, u1, u2, pm & 3);
+          A[k][kY] = pick3(u0, u1, u2, (pm >> 2) & 3);
+          A[k][kZ] = pick3(u0, u1, u2, (pm >> 4) & 3);
+          A[k][kD] = u3;
+        }
+        const double c = s_c[e];
+        if (ch != 1) {
+          double a0[4], a2[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) { a0[q] = c * A[0][q]; a2[q] = c * A[2][q]; }
+          const double sx = ch == 0 ? 1.0 : -1.0;
+          ladder(acc, a0, A[1], sx);
+          ladder(acc, a2, A[3], sx);
+        } else {
+          // chalice with Cb / Cb2, inverse chalice with ICa / ICa2: bundle k
+          // meets the factors of local vertex k
+          const double *fl = s_loc + e * 16;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc[q] = fma(A[3][q], fl[12 + q], fma(A[2][q], fl[8 + q], fma(A[1][q], fl[4 + q], fma(A[0][q], fl[q], acc[q]))));
+          // lattice-bubble factors of the first pair: P = c A0, Q = A1
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            s_P[(q * nb + e) * Lp + rid] = c * A[0][q];
+            s_Q[(q * nb + e) * Lp + rid] = A[1][q];
           }
         }
       }
-      __syncthreads();
-      if (ch == 1) {
-        // (C) M[k] += sum_e P[a_k][e] (x) Q[b_k][e], 2x2 register tiles
-        const int hp = Lp >> 1;
-        const int tiles = npairs * hp * hp;
-        for (int j = tid; j < tiles; j += kXyzThreads) {
-          const int k = j / (hp * hp);
-          const int r = j - k * hp * hp;
-          const int a = 2 * (r / hp), b = 2 * (r - (r / hp) * hp);
-          const int pa = F.pair[k] >> 2, pbq = F.pair[k] & 3;
-          const double *P = s_P + pa * nb * Lp + a;
-          const double *Q = s_Q + pbq * nb * Lp + b;
-          double *M0 = s_M + k * Lp * Lp + a * Lp + b;
-          double *M1 = M0 + Lp;
-          double2 c0 = *reinterpret_cast<double2 *>(M0), c1 = *reinterpret_cast<double2 *>(M1);
-          for (int e = 0; e < nvalid; ++e) {
-            const double2 p = *reinterpret_cast<const double2 *>(P + e * Lp);
-            const double2 q = *reinterpret_cast<const double2 *>(Q + e * Lp);
-            c0.x = fma(p.x, q.x, c0.x);
-            c0.y = fma(p.x, q.y, c0.y);
-            c1.x = fma(p.y, q.x, c1.x);
-            c1.y = fma(p.y, q.y, c1.y);
+    }
+    __syncthreads();
+    if (jc >= 0 && ch == 1) {
+      // (C) M[k] += sum_e P[a_k][e] (x) Q[b_k][e], 4x4 register tiles
+      const int hq = Lp >> 2;
+      const int tiles = npairs * hq * hq;
+      for (int jt = tid; jt < tiles; jt += kXyzPW) {
+        const int k = jt / (hq * hq);
+        const int r = jt - k * hq * hq;
+        const int a = 4 * (r / hq), b = 4 * (r - (r / hq) * hq);
+        const int pa = F.pair[k] >> 2, pbq = F.pair[k] & 3;
+        const double *P = s_P + pa * nb * Lp + a;
+        const double *Q = s_Q + pbq * nb * Lp + b;
+        double *M = s_M + k * Lp * Lp + a * Lp + b;
+        double2 m[4][2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          m[i][0] = *reinterpret_cast<const double2 *>(M + i * Lp);
+          m[i][1] = *reinterpret_cast<const double2 *>(M + i * Lp + 2);
+        }
+        for (int e = 0; e < nvalid; ++e) {
+          const double2 p01 = *reinterpret_cast<const double2 *>(P + e * Lp);
+          const double2 p23 = *reinterpret_cast<const double2 *>(P + e * Lp + 2);
+          const double2 q01 = *reinterpret_cast<const double2 *>(Q + e * Lp);
+          const double2 q23 = *reinterpret_cast<const double2 *>(Q + e * Lp + 2);
+          const double pv[4] = {p01.x, p01.y, p23.x, p23.y};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            m[i][0].x = fma(pv[i], q01.x, m[i][0].x);
+            m[i][0].y = fma(pv[i], q01.y, m[i][0].y);
+            m[i][1].x = fma(pv[i], q23.x, m[i][1].x);
+            m[i][1].y = fma(pv[i], q23.y, m[i][1].y);
           }
-          *reinterpret_cast<double2 *>(M0) = c0;
-          *reinterpret_cast<double2 *>(M1) = c1;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          *reinterpret_cast<double2 *>(M + i * Lp) = m[i][0];
+          *reinterpret_cast<double2 *>(M + i * Lp + 2) = m[i][1];
+        }
+      }
+      __syncthreads();
+      if (jc == cS + cT - 1) {
+        // lattice bubble (XYZFrgCore.cpp:295-322): RPA_c(rid) = sum over
+        // getOverlap(rid) of M[pair of the entry's transforms of c][rid1][rid2], x 4
+        const int wave = tid >> 6;
+        for (int r = wave; r < L; r += kXyzPW / 64) {
+          double sx = 0.0, sy = 0.0, sz = 0.0, sd = 0.0;
+          const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
+          for (int i = b + lane; i < e; i += 64) {
+            const int4 m = F.ovl_midx[i];
+            sx += s_M[m.x];
+            sy += s_M[m.y];
+            sz += s_M[m.z];
+            sd += s_M[m.w];
+          }
+          sx = wave_sum(sx);
+          sy = wave_sum(sy);
+          sz = wave_sum(sz);
+          sd = wave_sum(sd);
+          if (lane == 0) {
+            s_rpa[4 * r + 0] = 4.0 * sx;
+            s_rpa[4 * r + 1] = 4.0 * sy;
+            s_rpa[4 * r + 2] = 4.0 * sz;
+            s_rpa[4 * r + 3] = 4.0 * sd;
+          }
         }
         __syncthreads();
       }
-    }
-    if (ch == 1) {
-      // lattice bubble (XYZFrgCore.cpp:295-322): RPA_c(rid) = sum over
-      // getOverlap(rid) of M[pair of the entry's transforms of c][rid1][rid2], x 4
-      const int wave = tid >> 6, lane = tid & 63;
-      for (int r = wave; r < L; r += kXyzThreads / 64) {
-        double sx = 0.0, sy = 0.0, sz = 0.0, sd = 0.0;
-        const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
-        for (int i = b + lane; i < e; i += 64) {
-          const int4 m = F.ovl_midx[i];
-          sx += s_M[m.x];
-          sy += s_M[m.y];
-          sz += s_M[m.z];
-          sd += s_M[m.w];
-        }
-        sx = wave_sum(sx);
-        sy = wave_sum(sy);
-        sz = wave_sum(sz);
-        sd = wave_sum(sd);
-        if (lane == 0) {
-          s_rpa[4 * r + 0] = 4.0 * sx;
-          s_rpa[4 * r + 1] = 4.0 * sy;
-          s_rpa[4 * r + 2] = 4.0 * sz;
-          s_rpa[4 * r + 3] = 4.0 * sd;
-        }
-      }
-      __syncthreads();
     }
   }
 
@@ -488,7 +553,7 @@ __global__ __launch_bounds__(kXyzThreads) void k_xyz_dgamma(XyzDev F) {
     for (int k = 0; k < 4; ++k) s_red[(slot * L + rid) * 4 + k] = acc[k];
   }
   __syncthreads();
-  for (int q = tid; q < 4 * L; q += kXyzThreads) {
+  for (int q = tid; q < 4 * L; q += kXyzPW) {
     const int r = q >> 2, k = q & 3;
     double v = s_rpa[q];
     for (int sl = 0; sl < slots; ++sl) v += s_red[(sl * L + r) * 4 + k];
@@ -982,7 +1047,7 @@ hipError_t xyz_launch_compute(sp_xyz_ctx *ctx) {
   if (ctx->direct)
     hipLaunchKernelGGL(k_xyz_dgamma_direct, grid, dim3(kXyzThreads), ctx->lds, ctx->stream, F);
   else
-    hipLaunchKernelGGL(k_xyz_dgamma, grid, dim3(kXyzThreads), ctx->lds, ctx->stream, F);
+    hipLaunchKernelGGL(k_xyz_dgamma, grid, dim3(kXyzPW), ctx->lds, ctx->stream, F);
   return hipGetLastError();
 }
 
@@ -1049,7 +1114,7 @@ int sp_xyz_ctx_create(const sp_xyz_problem *p, sp_xyz_ctx **out) {
   use_pair(kD, kD);
   for (int i = 0; i < O; ++i)
     for (int c = 0; c < 3; ++c) use_pair(p->ovl_perm1[3 * i + c], p->ovl_perm2[3 * i + c]);
-  const int Lp = p->L + (p->L & 1);
+  const int Lp = (p->L + 3) & ~3;  // row stride of the bubble matrices
   std::vector<int4> midx(O > 0 ? O : 1);
   for (int i = 0; i < O; ++i) {
     const int base = p->ovl_rid1[i] * Lp + p->ovl_rid2[i];
@@ -1060,9 +1125,11 @@ int sp_xyz_ctx_create(const sp_xyz_problem *p, sp_xyz_ctx **out) {
   }
   int max_lds = 0;
   SP_XHIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
-  // rank-update kernel: (slot, rid) threads, one chunk = one point per slot
-  f.slots = kXyzThreads / p->L;
-  f.nb = f.slots;
+  // producer/worker kernel: (slot, rid) worker threads, kXyzPer points per
+  // slot and chunk; the producer wave holds one lane per (point, pair)
+  f.slots = (kXyzPW - 64) / p->L;
+  f.nb = f.slots * kXyzPer;
+  if (f.nb > 32) f.nb = 32;
   ctx->lds = (size_t)xyz_lds(f.nw, f.L, f.nb, f.slots, f.npairs).total;
   const char *force = std::getenv("SPFRG_XYZ_BUBBLE");
   ctx->direct = ctx->lds > (size_t)max_lds || (force && std::strcmp(force, "direct") == 0);
