@@ -92,6 +92,16 @@ constexpr int kMaxBatchMixed = SPFRG_MAX_BATCH_MIXED;
 #ifndef SPFRG_UNROLL_MIXED
 #define SPFRG_UNROLL_MIXED 1
 #endif
+// register tile of the FP64 lattice-bubble rank update (rows x columns; both
+// divide 4, the row stride of the bubble matrices).  2 x 4 keeps 144 of the
+// 256 threads busy at L = 23 with 3 LDS reads per 8 FMAs (2 x 2: 2 rounds of
+// 4 FMAs per 2 reads; 4 x 4: only 72 threads)
+#ifndef SPFRG_TILE_R
+#define SPFRG_TILE_R 2
+#endif
+#ifndef SPFRG_TILE_C
+#define SPFRG_TILE_C 4
+#endif
 #ifndef SPFRG_CHUNK
 #define SPFRG_CHUNK 16
 #endif
@@ -159,7 +169,7 @@ struct FlowDev {
   double *v2, *dv2;
   const double *mesh;
   const int *site_rid, *inv_rid, *ovl_off, *ovl_rid1, *ovl_rid2, *range_fwd, *range_inv, *corr_rid;
-  const unsigned short *ovl_idx;  // per overlap entry: rid1 * Lp + rid2 (Lp = L rounded up to even)
+  const unsigned short *ovl_idx;  // per overlap entry: rid1 * Lp + rid2 (Lp = L rounded up to a multiple of 4)
   double *Wwp, *Wc;
   int *Wn;
   double *Cwp, *Cw;
@@ -234,7 +244,7 @@ __global__ __launch_bounds__(64) void k_dsigma(const FlowDev *__restrict__ fl) {
 // dGamma/dLambda
 // ---------------------------------------------------------------------------
 // LDS carve-up (bytes), computed identically on host and device.  Lp = L
-// rounded up to even (2x2 register tiles of the lattice-bubble update); vs =
+// rounded up to a multiple of 4 (register tiles of the lattice-bubble update); vs =
 // size of the gather type (8 FP64, 4 mixed).  The per-chunk descriptors (c,
 // off, w, ex, loc) are double-buffered: the producer wave fills one buffer
 // while the worker waves read the other.
@@ -244,7 +254,7 @@ struct LdsLayout {
 
 __host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots, int vs) {
   LdsLayout l;
-  const int Lp = L + (L & 1);
+  const int Lp = (L + 3) & ~3;
   int off = 0;
   auto take = [&](int bytes) { int o = off; off += (bytes + 15) & ~15; return o; };
   l.mesh = take(nw * 8);
@@ -314,7 +324,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
   const int item = (int)(lb - F.item_base);
 
   const int nw = F.nw, L = F.L, nb = F.nb, slots = F.slots;
-  const int Lp = L + (L & 1);
+  const int Lp = (L + 3) & ~3;
   const LdsLayout lay = lds_layout(nw, L, nb, slots, (int)sizeof(V));
   double *s_mesh = reinterpret_cast<double *>(lds_raw + lay.mesh);
   V *s_P = reinterpret_cast<V *>(lds_raw + lay.P);
@@ -589,18 +599,19 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
     }
     __syncthreads();
     if (jc >= 0 && ch == 1) {
-      // (C) C[ch][a][b] += sum_e P[ch][e][a] Q[ch][e][b], 2x2 register tiles
-      const int hp = Lp >> 1;
-      const int tiles = 2 * hp * hp;
-      for (int jt = tid; jt < tiles; jt += kThreads) {
-        const int cch = jt / (hp * hp);
-        const int r = jt - cch * hp * hp;
-        const int a = 2 * (r / hp), b = 2 * (r - (r / hp) * hp);
-        const V *P = s_P + cch * nb * Lp + a;
-        const V *Q = s_Q + cch * nb * Lp + b;
-        double *C0 = s_C + cch * Lp * Lp + a * Lp + b;
-        double *C1 = s_C + cch * Lp * Lp + (a + 1) * Lp + b;
-        if (MIXED) {
+      // (C) C[ch][a][b] += sum_e P[ch][e][a] Q[ch][e][b]: FP64 in TR x TC
+      // register tiles, the compensated FP32 sums of the mixed form in 2x2
+      if (MIXED) {
+        const int hp = Lp >> 1;
+        const int tiles = 2 * hp * hp;
+        for (int jt = tid; jt < tiles; jt += kThreads) {
+          const int cch = jt / (hp * hp);
+          const int r = jt - cch * hp * hp;
+          const int a = 2 * (r / hp), b = 2 * (r - (r / hp) * hp);
+          const V *P = s_P + cch * nb * Lp + a;
+          const V *Q = s_Q + cch * nb * Lp + b;
+          double *C0 = s_C + cch * Lp * Lp + a * Lp + b;
+          double *C1 = s_C + cch * Lp * Lp + (a + 1) * Lp + b;
           KSum c00, c01, c10, c11;
           __builtin_memcpy(&c00, C0, 8);
           __builtin_memcpy(&c01, C0 + 1, 8);
@@ -618,18 +629,43 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
           __builtin_memcpy(C0 + 1, &c01, 8);
           __builtin_memcpy(C1, &c10, 8);
           __builtin_memcpy(C1 + 1, &c11, 8);
-        } else {
-          double2 c0 = *reinterpret_cast<double2 *>(C0), c1 = *reinterpret_cast<double2 *>(C1);
+        }
+      } else {
+        constexpr int TR = SPFRG_TILE_R, TC = SPFRG_TILE_C;
+        const int nr = Lp / TR, nc = Lp / TC;
+        const int tiles = 2 * nr * nc;
+        for (int jt = tid; jt < tiles; jt += kThreads) {
+          const int cch = jt / (nr * nc);
+          const int r = jt - cch * nr * nc;
+          const int a = TR * (r / nc), b = TC * (r - (r / nc) * nc);
+          const double *P = reinterpret_cast<const double *>(s_P) + cch * nb * Lp + a;
+          const double *Q = reinterpret_cast<const double *>(s_Q) + cch * nb * Lp + b;
+          double *Cb = s_C + cch * Lp * Lp + a * Lp + b;
+          double2 m[TR][TC / 2];
+#pragma unroll
+          for (int i = 0; i < TR; ++i)
+#pragma unroll
+            for (int jj = 0; jj < TC / 2; ++jj) m[i][jj] = *reinterpret_cast<const double2 *>(Cb + i * Lp + 2 * jj);
           for (int e = 0; e < nvalid; ++e) {
-            const V2 p = *reinterpret_cast<const V2 *>(P + e * Lp);
-            const V2 q = *reinterpret_cast<const V2 *>(Q + e * Lp);
-            c0.x = fma(double(p.x), double(q.x), c0.x);
-            c0.y = fma(double(p.x), double(q.y), c0.y);
-            c1.x = fma(double(p.y), double(q.x), c1.x);
-            c1.y = fma(double(p.y), double(q.y), c1.y);
+            double2 pv[TR / 2], qv[TC / 2];
+#pragma unroll
+            for (int i = 0; i < TR / 2; ++i) pv[i] = *reinterpret_cast<const double2 *>(P + e * Lp + 2 * i);
+#pragma unroll
+            for (int jj = 0; jj < TC / 2; ++jj) qv[jj] = *reinterpret_cast<const double2 *>(Q + e * Lp + 2 * jj);
+#pragma unroll
+            for (int i = 0; i < TR; ++i) {
+              const double pi_ = (i & 1) ? pv[i >> 1].y : pv[i >> 1].x;
+#pragma unroll
+              for (int jj = 0; jj < TC / 2; ++jj) {
+                m[i][jj].x = fma(pi_, qv[jj].x, m[i][jj].x);
+                m[i][jj].y = fma(pi_, qv[jj].y, m[i][jj].y);
+              }
+            }
           }
-          *reinterpret_cast<double2 *>(C0) = c0;
-          *reinterpret_cast<double2 *>(C1) = c1;
+#pragma unroll
+          for (int i = 0; i < TR; ++i)
+#pragma unroll
+            for (int jj = 0; jj < TC / 2; ++jj) *reinterpret_cast<double2 *>(Cb + i * Lp + 2 * jj) = m[i][jj];
         }
       }
       __syncthreads();
@@ -1131,7 +1167,7 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
     SP_HIP(hipMemcpy(o1, p->ovl_rid1, (size_t)O * 4, hipMemcpyHostToDevice));
     SP_HIP(hipMemcpy(o2, p->ovl_rid2, (size_t)O * 4, hipMemcpyHostToDevice));
     // packed C-matrix index of every overlap entry (rid < L <= 64, so it fits 16 bits)
-    const int Lp = f.L + (f.L & 1);
+    const int Lp = (f.L + 3) & ~3;
     std::vector<unsigned short> idx(O);
     for (int i = 0; i < O; ++i) idx[i] = (unsigned short)(p->ovl_rid1[i] * Lp + p->ovl_rid2[i]);
     SP_HIP(hipMemcpy(oidx, idx.data(), (size_t)O * 2, hipMemcpyHostToDevice));
