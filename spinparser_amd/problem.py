@@ -213,11 +213,13 @@ class TRIProblem:
         Per integrand call: 4 bundled gathers of 16 L values (2 FLOP per support
         x 4 supports: 512 L), the channel algebra (S, U: 2 x 256 products, 1024 L;
         T: chalice + inverse chalice 2 x 2 x 256 L products, 2048 L, their 4 local
-        vertices 4 x 16 x 8 FLOP, and the lattice bubble 128 O products, 512 O).
-        Euler 2 |Gamma|."""
+        vertices 4 x 16 x 8 FLOP, and the lattice bubble 128 O products, 256 O:
+        the 128 statements `RPA += 2 * a * b` per overlap entry of
+        src/TRI/TRIFrgCore.cpp:739-1252, one FMA each with the 2 folded, as the
+        SU(2) and XYZ bubbles are counted).  Euler 2 |Gamma|."""
         L, O = self.L, self.overlap_total
         c_su = 512.0 * L + 1024.0 * L
-        c_t = 512.0 * L + 2048.0 * L + 512.0 + 512.0 * O
+        c_t = 512.0 * L + 2048.0 * L + 512.0 + 256.0 * O
         return c_su * (calls_s + calls_u) + c_t * calls_t + 2.0 * self.vertex_size + 2.0 * self.nw
 
 
