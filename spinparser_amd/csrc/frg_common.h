@@ -152,37 +152,56 @@ __device__ inline void katanin_windows(const double *m, int n, double X, double 
 // (src/SU2/SU2FrgCore.cpp:349-371, src/TRI/TRIFrgCore.cpp:2917-2940), then the
 // Katanin windows with dSigma(w') G(w')^2 G(X + w') folded into the trapezoid
 // weight (src/SU2/SU2FrgCore.cpp:343-347,373-424).  Depends on (x, cut) only,
-// so every item of a flow shares the table.  *overflow is raised if the list
-// exceeds emax entries.
-__device__ inline void channel_points(int x, const double *m, int nw, double cut, const double *v2, const double *dv2,
-                                      double *Wwp, double *Wc, int *Wn, int emax, int *overflow) {
-  const double X = m[x];
-  PointList o{Wwp + (size_t)x * emax, Wc + (size_t)x * emax, 0, emax};
-  auto p = [&](double a, double b) { return 1.0 / ((a + sigma_at(m, nw, v2, a)) * (b + sigma_at(m, nw, v2, b))); };
-  o.push(cut, p(cut, cut + X));
-  if (X > 2.0 * cut) o.push(-cut, p(cut, cut - X));
-  const int k0 = o.cnt;
-  katanin_windows(m, nw, X, cut, o);
-  const int kend = o.cnt < emax ? o.cnt : emax;
-  for (int k = k0; k < kend; ++k) {
-    const double wp = o.wp[k];
-    const double d1 = wp + sigma_at(m, nw, v2, wp);
-    o.w[k] *= sigma_at(m, nw, dv2, wp) / (d1 * d1 * (X + wp + sigma_at(m, nw, v2, X + wp)));
+// so every item of a flow shares the table.
+//
+// weight_tables_wave: one 64-lane workgroup per table, x < nw a channel list,
+// x == nw the plain trapezoid list of the correlation integrals at nu = 0
+// (src/SU2/SU2MeasurementCorrelation.cpp:145-159,
+// src/TRI/TRIMeasurementCorrelation.cpp:270-284).  s_m is LDS space for the
+// nw mesh values; lane 0 builds the point list (the window walks are
+// sequential), then the lanes fold the Katanin weights of its points in
+// parallel.  *overflow is raised if a list exceeds emax entries.
+__device__ inline void weight_tables_wave(int x, const double *mesh, int nw, double cut, const double *v2,
+                                          const double *dv2, double *Wwp, double *Wc, int *Wn, double *Cwp,
+                                          double *Cw, int *Cn, int emax, int *overflow, double *s_m) {
+  __shared__ int s_k[2];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < nw; i += 64) s_m[i] = mesh[i];
+  __syncthreads();
+  const double *m = s_m;
+  if (x == nw) {
+    if (lane == 0) {
+      PointList o{Cwp, Cw, 0, emax};
+      katanin_windows(m, nw, 0.0, cut, o);
+      *Cn = o.cnt < emax ? o.cnt : emax;
+      if (o.cnt > emax) atomicOr(overflow, 1);
+    }
+    return;
   }
-  Wn[x] = kend;
-  if (o.cnt > emax) atomicOr(overflow, 1);
+  const double X = m[x];
+  double *wp = Wwp + (size_t)x * emax, *w = Wc + (size_t)x * emax;
+  if (lane == 0) {
+    PointList o{wp, w, 0, emax};
+    auto p = [&](double a, double b) { return 1.0 / ((a + sigma_at(m, nw, v2, a)) * (b + sigma_at(m, nw, v2, b))); };
+    o.push(cut, p(cut, cut + X));
+    if (X > 2.0 * cut) o.push(-cut, p(cut, cut - X));
+    s_k[0] = o.cnt;
+    katanin_windows(m, nw, X, cut, o);
+    s_k[1] = o.cnt;
+  }
+  __syncthreads();  // the list in global memory is visible to the workgroup
+  const int k0 = s_k[0], cnt = s_k[1];
+  const int kend = cnt < emax ? cnt : emax;
+  for (int k = k0 + lane; k < kend; k += 64) {
+    const double e = wp[k];
+    const double d1 = e + sigma_at(m, nw, v2, e);
+    w[k] *= sigma_at(m, nw, dv2, e) / (d1 * d1 * (X + e + sigma_at(m, nw, v2, X + e)));
+  }
+  if (lane == 0) {
+    Wn[x] = kend;
+    if (cnt > emax) atomicOr(overflow, 1);
+  }
 }
-
-// Plain trapezoid points of the correlation integrals at nu = 0
-// (src/SU2/SU2MeasurementCorrelation.cpp:145-159, src/TRI/TRIMeasurementCorrelation.cpp:270-284)
-__device__ inline void correlation_points(const double *m, int nw, double cut, double *Cwp, double *Cw, int *Cn,
-                                          int emax, int *overflow) {
-  PointList o{Cwp, Cw, 0, emax};
-  katanin_windows(m, nw, 0.0, cut, o);
-  *Cn = o.cnt < emax ? o.cnt : emax;
-  if (o.cnt > emax) atomicOr(overflow, 1);
-}
-
 
 // ---------------------------------------------------------------------------
 // access-buffer descriptors (SU(2) and XYZ layouts: one element per (row, site))

@@ -193,18 +193,16 @@ __device__ __forceinline__ int flow_of(const FlowDev *__restrict__ fl, int nflow
   return lo;
 }
 
-// W table: x < nw -> channel lists with propagators; x == nw -> plain
-// correlation list.  flags[1] is raised if a list overflows its capacity.
-__global__ void k_weights(const FlowDev *__restrict__ fl, int corr_only) {
+// W tables: workgroup x < nw -> channel list with propagators; x == nw ->
+// plain correlation list (corr_only: that one alone, launched as workgroup 0).
+// flags[1] is raised if a list overflows its capacity.  Dynamic LDS: the mesh.
+__global__ __launch_bounds__(64) void k_weights(const FlowDev *__restrict__ fl, int corr_only) {
+  extern __shared__ double s_wmesh[];
   const FlowDev &F = fl[blockIdx.y];
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = corr_only ? F.nw : (int)blockIdx.x;
   if (x > F.nw) return;
-  if (x == F.nw) {
-    correlation_points(F.mesh, F.nw, F.cut, F.Cwp, F.Cw, F.Cn, F.emax, F.flags + 1);
-    return;
-  }
-  if (corr_only) return;
-  channel_points(x, F.mesh, F.nw, F.cut, F.v2, F.dv2, F.Wwp, F.Wc, F.Wn, F.emax, F.flags + 1);
+  weight_tables_wave(x, F.mesh, F.nw, F.cut, F.v2, F.dv2, F.Wwp, F.Wc, F.Wn, F.Cwp, F.Cw, F.Cn, F.emax, F.flags + 1,
+                     s_wmesh);
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,7 +1020,7 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
   }
   {
     ProfScope ps(prof, 2);
-    hipLaunchKernelGGL(k_weights, dim3((geo.maxnw + 64) / 64, n), dim3(64), 0, st, d, 0);
+    hipLaunchKernelGGL(k_weights, dim3(geo.maxnw + 1, n), dim3(64), geo.maxnw * sizeof(double), st, d, 0);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -1056,7 +1054,7 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
 
 hipError_t launch_measure(hipStream_t st, const FlowDev *d, const FlowDev *h, int n) {
   const Geometry geo = geometry(h, n);
-  hipLaunchKernelGGL(k_weights, dim3((geo.maxnw + 64) / 64, n), dim3(64), 0, st, d, 1);
+  hipLaunchKernelGGL(k_weights, dim3(1, n), dim3(64), geo.maxnw * sizeof(double), st, d, 1);
   hipLaunchKernelGGL(k_corr_partial, dim3(geo.maxemax, n), dim3(kThreads), 0, st, d);
   hipLaunchKernelGGL(k_corr_final, dim3(1, n), dim3(256), 0, st, d);
   return hipGetLastError();

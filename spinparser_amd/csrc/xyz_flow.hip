@@ -107,15 +107,11 @@ __device__ __forceinline__ double gload(__amdgpu_buffer_rsrc_t r, int byte_off) 
 }
 
 // weight tables of one flow (the SU(2) core's definition)
-__global__ void k_xyz_weights(XyzDev F, int corr_only) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x > F.nw) return;
-  if (x == F.nw) {
-    correlation_points(F.mesh, F.nw, F.cut, F.Cwp, F.Cw, F.Cn, F.emax, F.flags + 1);
-    return;
-  }
-  if (corr_only) return;
-  channel_points(x, F.mesh, F.nw, F.cut, F.v2, F.dv2, F.Wwp, F.Wc, F.Wn, F.emax, F.flags + 1);
+__global__ __launch_bounds__(64) void k_xyz_weights(XyzDev F, int corr_only) {
+  extern __shared__ double s_wmesh[];
+  const int x = corr_only ? F.nw : (int)blockIdx.x;
+  weight_tables_wave(x, F.mesh, F.nw, F.cut, F.v2, F.dv2, F.Wwp, F.Wc, F.Wn, F.Cwp, F.Cw, F.Cn, F.emax, F.flags + 1,
+                     s_wmesh);
 }
 
 // getValue(i1, i2, s, t, u, c, FrequencyChannel::None) (XYZVertexTwoParticle.hpp:188-310)
@@ -1037,7 +1033,7 @@ hipError_t xyz_launch_compute(sp_xyz_ctx *ctx) {
   }
   {
     XyzProf ps(ctx, 2);
-    hipLaunchKernelGGL(k_xyz_weights, dim3((F.nw + 64) / 64), dim3(64), 0, ctx->stream, F, 0);
+    hipLaunchKernelGGL(k_xyz_weights, dim3(F.nw + 1), dim3(64), F.nw * sizeof(double), ctx->stream, F, 0);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -1300,7 +1296,7 @@ int sp_xyz_measure_correlation(sp_xyz_ctx *ctx, double *out) {
   if (!ctx || !out) return SP_EARG;
   XyzDev F = ctx->fd;
   F.cut = ctx->cutoff;
-  hipLaunchKernelGGL(k_xyz_weights, dim3((F.nw + 64) / 64), dim3(64), 0, ctx->stream, F, 1);
+  hipLaunchKernelGGL(k_xyz_weights, dim3(1), dim3(64), F.nw * sizeof(double), ctx->stream, F, 1);
   hipLaunchKernelGGL(k_xyz_corr_partial, dim3(F.emax), dim3(kXyzThreads), 0, ctx->stream, F);
   hipLaunchKernelGGL(k_xyz_corr_final, dim3(1), dim3(256), 0, ctx->stream, F);
   SP_XHIP(hipGetLastError());
