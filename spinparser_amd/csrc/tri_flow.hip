@@ -32,11 +32,14 @@
 //      spin permutation);
 //   C  one thread per (point, site, buffer pair) applies the channel algebra:
 //      S / U 256 products per pair; T the lattice bubble (per overlap entry
-//      64 products of spin-permuted bundle entries), the chalice and inverse
+//      64 products; the entries of a site grouped by their spin
+//      transformations, so that a group's permutations become row addresses
+//      and one output permutation per group), the chalice and inverse
 //      chalice (256 products each), accumulated with the point's weight.
 // The per-thread sums are reduced through LDS at the end of the item.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -188,6 +191,10 @@ __device__ __forceinline__ void apply_table(double (&acc)[16], const double (&x)
 
 __device__ __forceinline__ int zeta(int s) { return s <= 2 ? -1 : 1; }
 
+__device__ __forceinline__ double pick3(double u0, double u1, double u2, int i) {
+  return i == 0 ? u0 : (i == 1 ? u1 : u2);
+}
+
 // ---------------------------------------------------------------------------
 // per-flow descriptor (passed by value)
 // ---------------------------------------------------------------------------
@@ -195,7 +202,12 @@ struct TriDev {
   double *G, *dG, *v2, *dv2;
   const double *mesh;
   const int *site_rid, *site_perm, *inv_rid, *inv_perm;
-  const int *ovl_off, *ovl_rid1, *ovl_rid2, *ovl_perm1, *ovl_perm2;
+  // lattice-bubble tables: the overlap entries of each site grouped by their
+  // spin transformations; bgrp_off[rid] .. bgrp_off[rid + 1] index bgrp =
+  // {first entry, end entry, P1 | Q << 6 | P2 << 12 (2 bits per spin index), 0};
+  // bent = rid1 | rid2 << 16 per entry
+  const int *bgrp_off, *bent;
+  const int4 *bgrp;
   const int *range_fwd, *range_inv, *corr_rid, *corr_perm;
   double *Wwp, *Wc;
   int *Wn;
@@ -523,34 +535,54 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
           else apply_table<1>(acc, x, y);
         } else {
           // lattice bubble (TRIFrgCore.cpp:733-1254): entry i of getOverlap(rid)
-          // pairs site rid1 of the first bundle and rid2 of the second, each
-          // with its spin transformation
-          const int ib = F.ovl_off[cj], ie = F.ovl_off[cj + 1];
+          // pairs site rid1 of the first bundle A and rid2 of the second B with
+          // their spin transformations P1, P2 (density index fixed):
+          //   out(rr, tt) += 2 g sum_nn sgn(rr, nn, tt) A(P1 rr, P1 nn) B(P2 nn, P2 tt).
+          // The entries of a site come grouped by (P1, P2).  With m = P1 nn and
+          // Q = P2 P1^-1 an entry adds
+          //   o(rr, c) += sum_m sgn(rr, m, c) A(P1 rr, m) B(Q m, c)
+          // (the sign depends only on which indices are the density, which
+          // spin permutations keep), so rows are picked by address; the group
+          // then adds 2 g o(rr, P2 tt) to out(rr, tt).
+          const int gb = F.bgrp_off[cj], ge = F.bgrp_off[cj + 1];
           const double c2 = 2.0 * cw;
-          for (int i = ib; i < ie; ++i) {
-            const int r1 = F.ovl_rid1[i], r2 = F.ovl_rid2[i];
-            int P1[4], P2[4];
+          for (int gi = gb; gi < ge; ++gi) {
+            const int4 gr = F.bgrp[gi];
+            const int code = gr.z;
+            // component rows (4 components, stride L) of A and B
+            const int xr[4] = {4 * (code & 3) * L, 4 * ((code >> 2) & 3) * L, 4 * ((code >> 4) & 3) * L, 12 * L};
+            const int yr[4] = {4 * ((code >> 6) & 3) * L, 4 * ((code >> 8) & 3) * L, 4 * ((code >> 10) & 3) * L, 12 * L};
+            double o[16];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) { P1[k] = F.ovl_perm1[3 * i + k]; P2[k] = F.ovl_perm2[3 * i + k]; }
-            P1[3] = 3;
-            P2[3] = 3;
-            double a[16], b[16];
+            for (int k = 0; k < 16; ++k) o[k] = 0.0;
+            for (int i = gr.x; i < gr.y; ++i) {
+              const int pk = F.bent[i];
+              const double *xa = X + (pk & 0xffff);
+              const double *ya = Y + (pk >> 16);
+              double bv[16];
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr)
+              for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-              for (int nn = 0; nn < 4; ++nn) {
-                a[4 * rr + nn] = X[(4 * P1[rr] + P1[nn]) * L + r1];
-                b[4 * rr + nn] = Y[(4 * P2[rr] + P2[nn]) * L + r2];
+                for (int cc = 0; cc < 4; ++cc) bv[4 * mm + cc] = ya[yr[mm] + cc * L];
+#pragma unroll
+              for (int rr = 0; rr < 4; ++rr) {
+                const double av[4] = {xa[xr[rr]], xa[xr[rr] + L], xa[xr[rr] + 2 * L], xa[xr[rr] + 3 * L]};
+#pragma unroll
+                for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+                  for (int cc = 0; cc < 4; ++cc)
+                    o[4 * rr + cc] = fma(bubble_sign(rr, mm, cc) > 0 ? av[mm] : -av[mm], bv[4 * mm + cc], o[4 * rr + cc]);
               }
+            }
+            const int p0 = (code >> 12) & 3, p1 = (code >> 14) & 3, p2 = (code >> 16) & 3;
 #pragma unroll
-            for (int rr = 0; rr < 4; ++rr)
-#pragma unroll
-              for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-                for (int nn = 0; nn < 4; ++nn) {
-                  const double f = bubble_sign(rr, nn, tt) > 0 ? c2 : -c2;
-                  acc[4 * rr + tt] = fma(f * a[4 * rr + nn], b[4 * nn + tt], acc[4 * rr + tt]);
-                }
+            for (int rr = 0; rr < 4; ++rr) {
+              const double *orow = o + 4 * rr;
+              acc[4 * rr + 0] = fma(c2, pick3(orow[0], orow[1], orow[2], p0), acc[4 * rr + 0]);
+              acc[4 * rr + 1] = fma(c2, pick3(orow[0], orow[1], orow[2], p1), acc[4 * rr + 1]);
+              acc[4 * rr + 2] = fma(c2, pick3(orow[0], orow[1], orow[2], p2), acc[4 * rr + 2]);
+              acc[4 * rr + 3] = fma(c2, orow[3], acc[4 * rr + 3]);
+            }
           }
           // chalice (pair 0: bundle 0 with local 4; pair 1: bundle 2 with local 5)
           const double *lc = s_loc + (ce * 4 + cpair) * 16;
@@ -848,6 +880,11 @@ int check_tri_problem(sp_tri_ctx *ctx, const sp_tri_problem *p) {
   if (!in_range(p->site_perm, 3L * p->L, 3) || !in_range(p->inv_perm, 3L * p->L, 3) ||
       !in_range(p->ovl_perm1, 3L * O, 3) || !in_range(p->ovl_perm2, 3L * O, 3) || !in_range(p->corr_perm, 3 * nc, 3))
     return tfail(ctx, SP_EARG, "spin permutation entry outside [0, 3)");
+  // the overlap transformations are symmetry operations: permutations of x, y, z
+  auto is_perm = [](const int *q) { return (1 << q[0] | 1 << q[1] | 1 << q[2]) == 7; };
+  for (int i = 0; i < O; ++i)
+    if (!is_perm(p->ovl_perm1 + 3 * i) || !is_perm(p->ovl_perm2 + 3 * i))
+      return tfail(ctx, SP_EARG, "overlap spin transformation is not a permutation of x, y, z");
   return SP_OK;
 }
 
@@ -884,7 +921,7 @@ int sp_tri_ctx_footprint(const sp_tri_problem *p, long long *bytes) {
   b += 3LL * p->nw * 8;                                                       // Sigma, dSigma, mesh
   b += 2 * (long long)p->nw * emax * 8 + 2 * emax * 8 + (p->nw + 1) * 4;      // weight tables
   b += emax * 10 * p->L * 8 + 10LL * p->B * p->Rb * 8 + 16;                   // correlation scratch, output, flags
-  b += 4LL * (8 * p->L + 1 + 8 * O + 2 * p->R + 4 * p->B * p->Rb);            // lattice tables
+  b += 4LL * (8 * p->L + 1 + 5 * O + 2 * p->R + 4 * p->B * p->Rb);            // lattice tables
   *bytes = b;
   return SP_OK;
 }
@@ -923,12 +960,42 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   const int emax = f.emax;
   const size_t nc = (size_t)f.B * f.Rb;
   double *mesh = nullptr;
-  int *site = nullptr, *sitep = nullptr, *inv = nullptr, *invp = nullptr, *ooff = nullptr, *o1 = nullptr,
-      *o2 = nullptr, *op1 = nullptr, *op2 = nullptr, *rf = nullptr, *ri = nullptr, *corr = nullptr, *corrp = nullptr;
+  // lattice-bubble tables: per site, the overlap entries stably ordered by
+  // their spin transformations (P1, P2) and cut into groups of equal (P1, P2)
+  std::vector<int> gofs(f.L + 1, 0), ent;
+  std::vector<int4> grp;
+  ent.reserve(O);
+  for (int r = 0; r < f.L; ++r) {
+    std::vector<std::pair<int, int>> keyed;  // (P1 code | P2 code << 6, entry)
+    for (int i = p->ovl_off[r]; i < p->ovl_off[r + 1]; ++i) {
+      const int *q1 = p->ovl_perm1 + 3 * i, *q2 = p->ovl_perm2 + 3 * i;
+      keyed.push_back({(q1[0] | q1[1] << 2 | q1[2] << 4) | (q2[0] | q2[1] << 2 | q2[2] << 4) << 6, i});
+    }
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+    for (size_t k = 0; k < keyed.size(); ++k) {
+      const int i = keyed[k].second;
+      if (k == 0 || keyed[k].first != keyed[k - 1].first) {
+        const int *q1 = p->ovl_perm1 + 3 * i, *q2 = p->ovl_perm2 + 3 * i;
+        int inv1[3];
+        for (int c = 0; c < 3; ++c) inv1[q1[c]] = c;
+        const int Q[3] = {q2[inv1[0]], q2[inv1[1]], q2[inv1[2]]};  // Q = P2 P1^-1
+        const int code = (q1[0] | q1[1] << 2 | q1[2] << 4) | (Q[0] | Q[1] << 2 | Q[2] << 4) << 6 |
+                         (q2[0] | q2[1] << 2 | q2[2] << 4) << 12;
+        grp.push_back(make_int4((int)ent.size(), (int)ent.size(), code, 0));
+      }
+      ent.push_back(p->ovl_rid1[i] | p->ovl_rid2[i] << 16);
+      grp.back().y = (int)ent.size();
+    }
+    gofs[r + 1] = (int)grp.size();
+  }
+  int *site = nullptr, *sitep = nullptr, *inv = nullptr, *invp = nullptr, *goff = nullptr, *bent = nullptr,
+      *rf = nullptr, *ri = nullptr, *corr = nullptr, *corrp = nullptr;
+  int4 *bgrp = nullptr;
   if ((rc = talloc(ctx, &mesh, f.nw)) || (rc = talloc(ctx, &site, f.L)) || (rc = talloc(ctx, &sitep, 3 * f.L)) ||
-      (rc = talloc(ctx, &inv, f.L)) || (rc = talloc(ctx, &invp, 3 * f.L)) || (rc = talloc(ctx, &ooff, f.L + 1)) ||
-      (rc = talloc(ctx, &o1, O)) || (rc = talloc(ctx, &o2, O)) || (rc = talloc(ctx, &op1, 3 * (size_t)O)) ||
-      (rc = talloc(ctx, &op2, 3 * (size_t)O)) || (rc = talloc(ctx, &rf, f.R)) || (rc = talloc(ctx, &ri, f.R)) ||
+      (rc = talloc(ctx, &inv, f.L)) || (rc = talloc(ctx, &invp, 3 * f.L)) || (rc = talloc(ctx, &goff, f.L + 1)) ||
+      (rc = talloc(ctx, &bent, ent.size())) || (rc = talloc(ctx, &bgrp, grp.size())) ||
+      (rc = talloc(ctx, &rf, f.R)) || (rc = talloc(ctx, &ri, f.R)) ||
       (rc = talloc(ctx, &corr, nc)) || (rc = talloc(ctx, &corrp, 3 * nc)) || (rc = talloc(ctx, &f.G, nv)) ||
       (rc = talloc(ctx, &f.dG, nv)) || (rc = talloc(ctx, &f.v2, f.nw)) || (rc = talloc(ctx, &f.dv2, f.nw)) ||
       (rc = talloc(ctx, &f.Wwp, (size_t)f.nw * emax)) || (rc = talloc(ctx, &f.Wc, (size_t)f.nw * emax)) ||
@@ -941,11 +1008,9 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   f.site_perm = sitep;
   f.inv_rid = inv;
   f.inv_perm = invp;
-  f.ovl_off = ooff;
-  f.ovl_rid1 = o1;
-  f.ovl_rid2 = o2;
-  f.ovl_perm1 = op1;
-  f.ovl_perm2 = op2;
+  f.bgrp_off = goff;
+  f.bent = bent;
+  f.bgrp = bgrp;
   f.range_fwd = rf;
   f.range_inv = ri;
   f.corr_rid = corr;
@@ -957,12 +1022,10 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   SP_THIP(hipMemcpy(sitep, p->site_perm, 3 * f.L * 4, hipMemcpyHostToDevice));
   SP_THIP(hipMemcpy(inv, p->inv_rid, f.L * 4, hipMemcpyHostToDevice));
   SP_THIP(hipMemcpy(invp, p->inv_perm, 3 * f.L * 4, hipMemcpyHostToDevice));
-  SP_THIP(hipMemcpy(ooff, p->ovl_off, (f.L + 1) * 4, hipMemcpyHostToDevice));
-  if (O > 0) {
-    SP_THIP(hipMemcpy(o1, p->ovl_rid1, (size_t)O * 4, hipMemcpyHostToDevice));
-    SP_THIP(hipMemcpy(o2, p->ovl_rid2, (size_t)O * 4, hipMemcpyHostToDevice));
-    SP_THIP(hipMemcpy(op1, p->ovl_perm1, 3 * (size_t)O * 4, hipMemcpyHostToDevice));
-    SP_THIP(hipMemcpy(op2, p->ovl_perm2, 3 * (size_t)O * 4, hipMemcpyHostToDevice));
+  SP_THIP(hipMemcpy(goff, gofs.data(), (f.L + 1) * 4, hipMemcpyHostToDevice));
+  if (!ent.empty()) {
+    SP_THIP(hipMemcpy(bent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
+    SP_THIP(hipMemcpy(bgrp, grp.data(), grp.size() * sizeof(int4), hipMemcpyHostToDevice));
   }
   SP_THIP(hipMemcpy(rf, p->range_fwd, (size_t)f.R * 4, hipMemcpyHostToDevice));
   SP_THIP(hipMemcpy(ri, p->range_inv, (size_t)f.R * 4, hipMemcpyHostToDevice));
