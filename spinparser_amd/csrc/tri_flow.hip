@@ -25,11 +25,12 @@
 // XCD-chunked like the SU(2) kernel.  Per channel the evaluation points are
 // processed in chunks of nb points:
 //   A  one thread per (point, access buffer) builds the descriptor (4 supports,
-//      weights, exchange / sign flags); the T channel's four site-local
-//      vertices are evaluated here directly (16 components each);
+//      weights, exchange / sign flags), the T channel's four site-local
+//      vertices included;
 //   B  one thread per (point, buffer, site) gathers the 16-component bundle
 //      into LDS (4 supports x 16 loads, component offsets through the site's
-//      spin permutation);
+//      spin permutation); in the T channel one thread per (point, local
+//      vertex, component) evaluates the site-local vertices;
 //   C  one thread per (point, site, buffer pair) applies the channel algebra:
 //      S / U 256 products per pair; T the lattice bubble (per overlap entry
 //      64 products; the entries of a site grouped by their spin
@@ -464,34 +465,41 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
           else off[q] = tri_support(l1, l2, xo, nw, FS, fx);
           if (fx) fl |= 4 << q;
         }
-        if (k < 4) {
-          const int d = e * 4 + k;
+        // descriptor slot: bundles e * 4 + k, the T channel's site-local
+        // vertices after all bundles (nb * 4 + e * 4 + k - 4)
+        const int d = k < 4 ? e * 4 + k : nb * 4 + e * 4 + (k - 4);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) { s_off[d * 4 + q] = off[q]; s_w[d * 4 + q] = w[q]; }
-          s_fl[d] = fl;
-        } else {
-          // getValueLocal (TRIVertexTwoParticle.hpp:300-309): site 0; the sign
-          // table is indexed with the exchanged components
-#pragma unroll 4
-          for (int cc = 0; cc < 16; ++cc) {
-            int s1 = cc / 4, s2 = cc % 4;
-            if (exch) { const int q = s1; s1 = s2; s2 = q; }
-            const double tsign = tu ? double(zeta(s1) * zeta(s2)) : 1.0;
-            const double fsign = -double(zeta(exch ? s1 : s2));
-            double v = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const double sg = ((fl >> (2 + q)) & 1) ? tsign * fsign : tsign;
-              v = fma(sg * w[q], gload(rG, (off[q] + (4 * s1 + s2) * L) * 8), v);
-            }
-            s_loc[(e * 4 + (k - 4)) * 16 + cc] = v;
-          }
-        }
+        for (int q = 0; q < 4; ++q) { s_off[d * 4 + q] = off[q]; s_w[d * 4 + q] = w[q]; }
+        s_fl[d] = fl;
         if (k == 0) s_c[e] = Wc[e0 + e];
       }
       __syncthreads();
-      // (B) bundles: (point, buffer, site) -> 16 components into LDS
-      for (int q = tid; q < nvalid * 4 * L; q += kTriThreads) {
+      // (B) bundles: (point, buffer, site) -> 16 components into LDS; in the T
+      // channel also the site-local vertices, one thread per (point, local, component)
+      const int nbq = nvalid * 4 * L;
+      const int nq = nbq + (ch == 1 ? nvalid * 4 * 16 : 0);
+      for (int q = tid; q < nq; q += kTriThreads) {
+        if (q >= nbq) {
+          // getValueLocal (TRIVertexTwoParticle.hpp:300-309): site 0; the sign
+          // table is indexed with the exchanged components
+          const int lq = q - nbq;
+          const int dl = lq >> 4, cc = lq & 15;  // dl = e * 4 + local
+          const int d = nb * 4 + dl;
+          const int fl = s_fl[d];
+          const bool exch = fl & 1, tu = (fl >> 1) & 1;
+          int s1 = cc / 4, s2 = cc % 4;
+          if (exch) { const int x = s1; s1 = s2; s2 = x; }
+          const double tsign = tu ? double(zeta(s1) * zeta(s2)) : 1.0;
+          const double fsign = -double(zeta(exch ? s1 : s2));
+          double v = 0.0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double sg = ((fl >> (2 + r)) & 1) ? tsign * fsign : tsign;
+            v = fma(sg * s_w[d * 4 + r], gload(rG, (s_off[d * 4 + r] + (4 * s1 + s2) * L) * 8), v);
+          }
+          s_loc[dl * 16 + cc] = v;
+          continue;
+        }
         const int d = q / L, j = q - (q / L) * L;  // d = e * 4 + k
         const int fl = s_fl[d];
         const bool exch = fl & 1, tu = (fl >> 1) & 1;
