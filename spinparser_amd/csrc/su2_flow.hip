@@ -994,6 +994,15 @@ size_t lds_of(const FlowDev &f) {
   return (size_t)lds_layout(f.nw, f.L, f.nb, f.slots, f.precision == SP_PRECISION_MIXED ? 4 : 8).total;
 }
 
+// Flows of one dGamma launch share its dynamic LDS size, the largest of them;
+// a flow whose plan leaves room for 4 workgroups per CU (160 KB of LDS) must
+// not run at the occupancy of a larger one.  Launch class = precision x
+// (LDS plan within that quarter or not); a batch issues one launch per class.
+constexpr size_t kLdsQuarter = 160 * 1024 / 4;
+int launch_class(const FlowDev &f) {
+  return (f.precision == SP_PRECISION_MIXED ? 2 : 0) + (lds_of(f) > kLdsQuarter ? 1 : 0);
+}
+
 // Launch geometry of a set of flows held in a host array (h) whose device copy is d.
 struct Geometry {
   int maxnw = 0, maxemax = 0;
@@ -1010,8 +1019,8 @@ Geometry geometry(const FlowDev *h, int n) {
   return g;
 }
 
-// dSigma, weight tables and dGamma for n flows; h[i].item_base must be set
-// relative to the first flow of each precision class, FP64 flows first.
+// dSigma, weight tables and dGamma for n flows; the flows come ordered by
+// launch_class and h[i].item_base is relative to the first flow of its class.
 hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, int n, sp_ctx *prof) {
   const Geometry geo = geometry(h, n);
   {
@@ -1026,17 +1035,18 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
   if (e != hipSuccess) return e;
   ProfScope ps(prof, 0);
   int i0 = 0;
-  for (int pass = 0; pass < 2; ++pass) {
-    const bool mixed = pass == 1;
+  while (i0 < n) {
+    const int cls = launch_class(h[i0]);
+    const bool mixed = h[i0].precision == SP_PRECISION_MIXED;
     int i1 = i0;
     long long total = 0;
     size_t lds = 0;
-    while (i1 < n && (h[i1].precision == SP_PRECISION_MIXED) == mixed) {
+    while (i1 < n && launch_class(h[i1]) == cls) {
       total += h[i1].nf;
       lds = std::max(lds, lds_of(h[i1]));
       ++i1;
     }
-    if (i1 > i0) {
+    {
       const long long per_xcd = 0;  // unused by the chunked mapping
       const long long rounds = (total + 8 * kChunk - 1) / (8 * kChunk);
       const unsigned grid = (unsigned)(rounds * 8 * kChunk);
@@ -1382,9 +1392,9 @@ static int group_prepare(sp_group *grp, sp_ctx *const *ctxs, int n, size_t *out_
   if (n < 1 || n > grp->capacity || !ctxs) return gfail(grp, SP_EARG, "invalid flow count");
   // FP64 flows first, then mixed; item bases restart per precision class
   std::vector<int> order;
-  for (int pass = 0; pass < 2; ++pass)
+  for (int cls = 0; cls < 4; ++cls)
     for (int i = 0; i < n; ++i)
-      if (ctxs[i] && (ctxs[i]->fd.precision == SP_PRECISION_MIXED) == (pass == 1)) order.push_back(i);
+      if (ctxs[i] && launch_class(ctxs[i]->fd) == cls) order.push_back(i);
   if ((int)order.size() != n) return gfail(grp, SP_EARG, "null context in group");
   size_t need = 0;
   for (int i = 0; i < n; ++i) {
@@ -1409,7 +1419,7 @@ static int group_prepare(sp_group *grp, sp_ctx *const *ctxs, int n, size_t *out_
     sp_ctx *c = ctxs[order[j]];
     SP_GHIP(hipStreamSynchronize(c->stream));
     FlowDev f = c->fd;
-    if (f.precision != prev) { base = 0; prev = f.precision; }
+    if (launch_class(f) != prev) { base = 0; prev = launch_class(f); }
     f.item_base = base;
     base += f.nf;
     f.cut = c->cutoff;
