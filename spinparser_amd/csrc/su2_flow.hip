@@ -981,13 +981,19 @@ int check_problem(sp_ctx *ctx, const sp_su2_problem *p) {
 
 // worker slots (waves 0..2) and points per chunk; the producer wave holds one
 // lane per (point, pair), so nb <= 32
+// (points per slot and chunk shrink while the LDS plan misses a quarter of
+// the CU's 160 KB, so that 4 workgroups stay resident per CU)
 void launch_shape(int nw, int L, int vs, int &slots, int &nb, size_t &lds) {
   slots = (kThreads - 64) / L;
   int per = (vs == 4 ? kMaxBatchMixed : kMaxBatch) / slots;
   if (per < 1) per = 1;
-  nb = slots * per;
-  if (nb > 32) nb = 32;
-  lds = (size_t)lds_layout(nw, L, nb, slots, vs).total;
+  for (;;) {
+    nb = slots * per;
+    if (nb > 32) nb = 32;
+    lds = (size_t)lds_layout(nw, L, nb, slots, vs).total;
+    if (lds <= 160 * 1024 / 4 || per == 1) break;
+    --per;
+  }
 }
 
 size_t lds_of(const FlowDev &f) {
@@ -998,7 +1004,7 @@ size_t lds_of(const FlowDev &f) {
 // a flow whose plan leaves room for 4 workgroups per CU (160 KB of LDS) must
 // not run at the occupancy of a larger one.  Launch class = precision x
 // (LDS plan within that quarter or not); a batch issues one launch per class.
-constexpr size_t kLdsQuarter = 160 * 1024 / 4;
+constexpr size_t kLdsQuarter = 160 * 1024 / 4;  // as in launch_shape
 int launch_class(const FlowDev &f) {
   return (f.precision == SP_PRECISION_MIXED ? 2 : 0) + (lds_of(f) > kLdsQuarter ? 1 : 0);
 }
