@@ -48,6 +48,29 @@ def test_group_step_equals_single_flow():
                                                                            len(alone.cutoffs)))
 
 
+def test_group_step_two_lds_classes():
+    """A group whose flows need different LDS plans issues one dGamma launch per
+    class (cubic range 8, L = 41, N_w = 16: 41 KB even at one point per slot,
+    above the quarter of the CU's LDS; square range 4: 28 KB); every flow still
+    equals its single-flow run."""
+    from spinparser_amd.capi import SP_PRECISION_FP64
+    from spinparser_amd.serve import FlowRequest, SweepServer, run_alone
+    from helpers import exp_problem
+    big = exp_problem("cubic", 8, "cubic-j1j2heisenberg", {"j1": 1.0, "j2": 0.3}, 16, 0.002, 100.0, 100.0, 0.05, 0.9)
+    assert big.L == 41
+    probs = [cfg1_problem(nw=8), big]
+    reqs = [FlowRequest(i, p, 0.0) for i, p in enumerate(probs)]
+    srv = SweepServer(budget_bytes=1 << 34, max_flows=2, precision=SP_PRECISION_FP64)
+    grouped = srv.run(reqs)
+    srv.close()
+    assert srv.peak_flows == 2
+    for r in reqs:
+        alone = run_alone(r, SP_PRECISION_FP64)
+        w = _records_close(grouped[r.ident], alone, 1e-13)
+        _log("flow %d (L=%d nw=%d): group vs alone %.2g over %d records" % (r.ident, r.problem.L, r.problem.nw, w,
+                                                                           len(alone.cutoffs)))
+
+
 def test_mixed_precision_error_vs_fp64():
     from spinparser_amd.capi import SP_PRECISION_FP64, SP_PRECISION_MIXED, FlowContext
     from spinparser_amd.serve import FlowRequest, run_alone
