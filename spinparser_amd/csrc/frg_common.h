@@ -7,6 +7,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 namespace frg {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -226,6 +228,15 @@ __device__ __forceinline__ Interp interp_of(const double *m, int nw, double x) {
   return r;
 }
 
+// interpolateOffset from the bracket position c = count_le(m, n, w)
+__device__ __forceinline__ void interp_finish(const double *m, int n, int c, double w, Interp &r) {
+  if (!(w > m[0])) { r.lo = 0; r.hi = 0; r.b = 0.0; return; }
+  if (c >= n) { r.lo = n - 1; r.hi = n - 1; r.b = 0.0; return; }
+  r.lo = c - 1;
+  r.hi = c;
+  r.b = (w - m[r.lo]) / (m[r.hi] - m[r.lo]);
+}
+
 // two interpolateOffset calls whose bracket searches run in one binary-lifting
 // loop, so that the two chains of dependent mesh reads overlap; each result
 // equals interp_of(m, n, x)
@@ -236,15 +247,44 @@ __device__ __forceinline__ void interp_pair(const double *m, int n, double x0, d
     if (q0 <= n && m[q0 - 1] <= x0) p0 = q0;
     if (q1 <= n && m[q1 - 1] <= x1) p1 = q1;
   }
-  auto finish = [&](Interp &r, int c, double w) {
-    if (!(w > m[0])) { r.lo = 0; r.hi = 0; r.b = 0.0; return; }
-    if (c >= n) { r.lo = n - 1; r.hi = n - 1; r.b = 0.0; return; }
-    r.lo = c - 1;
-    r.hi = c;
-    r.b = (w - m[r.lo]) / (m[r.hi] - m[r.lo]);
-  };
-  finish(a, p0, x0);
-  finish(b, p1, x1);
+  interp_finish(m, n, p0, x0, a);
+  interp_finish(m, n, p1, x1, b);
+}
+
+// Bracket positions on a geometric mesh (the exponential discretisation,
+// m[i] close to m[0] r^i): the guess floor(log2(w) ga + gb) + 1 from the
+// hardware FP32 logarithm is within one of count_le, and the comparisons with
+// the mesh that follow make the position exact for any mesh.  ga = 1 / log2 r,
+// gb = -log2(m[0]) / log2 r (geo_mesh_params).
+__device__ __forceinline__ int count_le_geo(const double *m, int n, double w, float ga, float gb) {
+  float t = fmaf(__log2f((float)w), ga, gb) + 1.0f;
+  t = fminf(fmaxf(t, 0.0f), (float)n);  // NaN (w <= 0) -> 0
+  int c = (int)t;
+  while (c < n && m[c] <= w) ++c;
+  while (c > 0 && m[c - 1] > w) --c;
+  return c;
+}
+
+__device__ __forceinline__ void interp_pair_geo(const double *m, int n, double x0, double x1, float ga, float gb,
+                                                Interp &a, Interp &b) {
+  const int p0 = count_le_geo(m, n, x0, ga, gb), p1 = count_le_geo(m, n, x1, ga, gb);
+  interp_finish(m, n, p0, x0, a);
+  interp_finish(m, n, p1, x1, b);
+}
+
+// Host: parameters of count_le_geo when every log2(m[i] / m[0]) lies within a
+// quarter step of i log2 r (on = 0 otherwise: use the binary search).
+inline void geo_mesh_params(const double *m, int n, int &on, float &ga, float &gb) {
+  on = 0;
+  ga = gb = 0.0f;
+  if (n < 2 || !(m[0] > 0.0)) return;
+  const double l2r = std::log2(m[n - 1] / m[0]) / (n - 1);
+  if (!(l2r > 0.0) || !std::isfinite(l2r)) return;
+  for (int i = 0; i < n; ++i)
+    if (!(std::fabs(std::log2(m[i] / m[0]) - i * l2r) <= 0.25)) return;
+  on = 1;
+  ga = (float)(1.0 / l2r);
+  gb = (float)(-std::log2(m[0]) / l2r);
 }
 
 template <class V2> __device__ __forceinline__ V2 buf_load(__amdgpu_buffer_rsrc_t r, int byte_off);

@@ -181,6 +181,8 @@ struct FlowDev {
   unsigned g_bytes, g32_bytes;
   double spin, cut, step;
   int pad_index;  // host bookkeeping: position of the flow in the caller's array
+  int geo;        // geometric mesh: bracket guess from the logarithm (count_le_geo)
+  float ga, gb;
 };
 
 __device__ __forceinline__ int flow_of(const FlowDev *__restrict__ fl, int nflows, long long item) {
@@ -446,7 +448,8 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
             }
           }
           Interp i0, i1;
-          interp_pair(s_mesh, nw, x0, x1, i0, i1);
+          if (F.geo) interp_pair_geo(s_mesh, nw, x0, x1, F.ga, F.gb, i0, i1);
+          else interp_pair(s_mesh, nw, x0, x1, i0, i1);
           // fully unrolled with a guard so that pb[] stays in registers (a
           // dynamically indexed local array would live in scratch memory)
 #pragma unroll
@@ -1173,6 +1176,7 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
 
   std::vector<double> m(f.nw);
   for (int i = 0; i < f.nw; ++i) m[i] = (double)p->mesh[i];
+  geo_mesh_params(m.data(), f.nw, f.geo, f.ga, f.gb);
   SP_HIP(hipMemcpy(mesh, m.data(), f.nw * 8, hipMemcpyHostToDevice));
   SP_HIP(hipMemcpy(site, p->site_rid, f.L * 4, hipMemcpyHostToDevice));
   SP_HIP(hipMemcpy(inv, p->inv_rid, f.L * 4, hipMemcpyHostToDevice));
