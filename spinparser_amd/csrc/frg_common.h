@@ -272,6 +272,17 @@ __device__ __forceinline__ void interp_pair_geo(const double *m, int n, double x
   interp_finish(m, n, p1, x1, b);
 }
 
+// interpolateOffset with the bracket from count_le_geo (geo != 0) or from the
+// binary search
+__device__ __forceinline__ void interp_sel(const double *m, int n, double w, int geo, float ga, float gb, int &lo,
+                                           int &hi, double &bias) {
+  Interp r;
+  interp_finish(m, n, geo ? count_le_geo(m, n, w, ga, gb) : count_le(m, n, w), w, r);
+  lo = r.lo;
+  hi = r.hi;
+  bias = r.b;
+}
+
 // Host: parameters of count_le_geo when every log2(m[i] / m[0]) lies within a
 // quarter step of i log2 r (on = 0 otherwise: use the binary search).
 inline void geo_mesh_params(const double *m, int n, int &on, float &ga, float &gb) {

@@ -220,6 +220,8 @@ struct TriDev {
   long long nf, nv;
   unsigned g_bytes;
   double cut, step;
+  int geo;  // geometric mesh: bracket guess from the logarithm (count_le_geo)
+  float ga, gb;
 };
 
 // LDS stride of one 16 x L bundle (doubles).  The padding keeps bundle bases
@@ -449,9 +451,9 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
         a = fabs(a); b = fabs(b); c = fabs(c);
         int lo1, hi1, lo2, hi2;
         double b1, b2;
-        if (chan == 0) { interp(s_mesh, nw, b, lo1, hi1, b1); interp(s_mesh, nw, c, lo2, hi2, b2); }
-        else if (chan == 1) { interp(s_mesh, nw, a, lo1, hi1, b1); interp(s_mesh, nw, c, lo2, hi2, b2); }
-        else { interp(s_mesh, nw, a, lo1, hi1, b1); interp(s_mesh, nw, b, lo2, hi2, b2); }
+        const double x1 = chan == 0 ? b : a, x2 = chan == 2 ? b : c;
+        interp_sel(s_mesh, nw, x1, F.geo, F.ga, F.gb, lo1, hi1, b1);
+        interp_sel(s_mesh, nw, x2, F.geo, F.ga, F.gb, lo2, hi2, b2);
         int off[4];
         double w[4];
         int fl = (exch ? 1 : 0) | (tu ? 2 : 0);
@@ -1025,6 +1027,7 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   f.corr_perm = corrp;
   std::vector<double> m(f.nw);
   for (int i = 0; i < f.nw; ++i) m[i] = (double)p->mesh[i];
+  geo_mesh_params(m.data(), f.nw, f.geo, f.ga, f.gb);
   SP_THIP(hipMemcpy(mesh, m.data(), f.nw * 8, hipMemcpyHostToDevice));
   SP_THIP(hipMemcpy(site, p->site_rid, f.L * 4, hipMemcpyHostToDevice));
   SP_THIP(hipMemcpy(sitep, p->site_perm, 3 * f.L * 4, hipMemcpyHostToDevice));

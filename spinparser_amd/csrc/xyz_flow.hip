@@ -97,6 +97,8 @@ struct XyzDev {
   long long np;  // N_f * L
   unsigned g_bytes;
   double cut, step;
+  int geo;  // geometric mesh: bracket guess from the logarithm (count_le_geo)
+  float ga, gb;
 };
 
 __device__ __forceinline__ double gload(__amdgpu_buffer_rsrc_t r, int byte_off) {
@@ -373,7 +375,8 @@ __global__ __launch_bounds__(kXyzPW, SPFRG_XYZ_MIN_BLOCKS) void k_xyz_dgamma(Xyz
             }
           }
           Interp i0, i1;
-          interp_pair(s_mesh, nw, x0, x1, i0, i1);
+          if (F.geo) interp_pair_geo(s_mesh, nw, x0, x1, F.ga, F.gb, i0, i1);
+          else interp_pair(s_mesh, nw, x0, x1, i0, i1);
           // fully unrolled with a guard so that pb[] stays in registers
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
@@ -1177,6 +1180,7 @@ int sp_xyz_ctx_create(const sp_xyz_problem *p, sp_xyz_ctx **out) {
   f.corr_perm = corrp;
   std::vector<double> m(f.nw);
   for (int i = 0; i < f.nw; ++i) m[i] = (double)p->mesh[i];
+  geo_mesh_params(m.data(), f.nw, f.geo, f.ga, f.gb);
   SP_XHIP(hipMemcpy(mesh, m.data(), f.nw * 8, hipMemcpyHostToDevice));
   SP_XHIP(hipMemcpy(site, p->site_rid, f.L * 4, hipMemcpyHostToDevice));
   SP_XHIP(hipMemcpy(sitep, p->site_perm, 3 * f.L * 4, hipMemcpyHostToDevice));
