@@ -860,11 +860,23 @@ __global__ void k_corr_final(const FlowDev *__restrict__ fl) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+constexpr int kRing = 8;  // pinned staging slots of the descriptor uploads
+
 struct sp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   FlowDev fd{};             // host copy of the flow descriptor
   FlowDev *d_fd = nullptr;  // device copy used by single-flow launches
+  // descriptor uploads go through pinned slots, so that they are truly
+  // asynchronous; a slot is refilled only after the copy that read it ran
+  FlowDev *h_ring = nullptr;
+  hipEvent_t ring_ev[kRing] = {};
+  int ring_pos = 0;
+  // pinned copies of the last measurement and of the flags taken with it:
+  // isDiverged right after a measurement needs no second round trip
+  double *h_meas = nullptr;
+  int *h_flags = nullptr;
+  bool flags_cached = false;
   double cutoff = 0.0;
   bool have_flow = false;
   size_t lds = 0;
@@ -1090,7 +1102,12 @@ hipError_t launch_euler(hipStream_t st, const FlowDev *d, const FlowDev *h, int 
 }
 
 int upload_fd(sp_ctx *ctx) {
-  SP_HIP(hipMemcpyAsync(ctx->d_fd, &ctx->fd, sizeof(FlowDev), hipMemcpyHostToDevice, ctx->stream));
+  const int s = ctx->ring_pos;
+  ctx->ring_pos = (s + 1) % kRing;
+  SP_HIP(hipEventSynchronize(ctx->ring_ev[s]));
+  ctx->h_ring[s] = ctx->fd;
+  SP_HIP(hipMemcpyAsync(ctx->d_fd, ctx->h_ring + s, sizeof(FlowDev), hipMemcpyHostToDevice, ctx->stream));
+  SP_HIP(hipEventRecord(ctx->ring_ev[s], ctx->stream));
   return SP_OK;
 }
 
@@ -1162,6 +1179,16 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
       (rc = dalloc(ctx, &f.corr_out, (size_t)2 * f.B * f.Rb)) || (rc = dalloc(ctx, &f.flags, 4)) ||
       (rc = dalloc(ctx, &ctx->d_fd, 1)))
     return rc;
+  {
+    void *q = nullptr;
+    SP_HIP(hipHostMalloc(&q, kRing * sizeof(FlowDev), 0));
+    ctx->h_ring = static_cast<FlowDev *>(q);
+    SP_HIP(hipHostMalloc(&q, (size_t)2 * f.B * f.Rb * 8, 0));
+    ctx->h_meas = static_cast<double *>(q);
+    SP_HIP(hipHostMalloc(&q, 16, 0));
+    ctx->h_flags = static_cast<int *>(q);
+    for (int k = 0; k < kRing; ++k) SP_HIP(hipEventCreateWithFlags(&ctx->ring_ev[k], hipEventDisableTiming));
+  }
   if (f.precision == SP_PRECISION_MIXED && (rc = dalloc(ctx, &f.G32, nv))) return rc;
   f.mesh = mesh;
   f.site_rid = site;
@@ -1213,6 +1240,11 @@ int sp_ctx_destroy(sp_ctx *ctx) {
   for (int k = 0; k < 4; ++k)
     for (auto &pr : ctx->pending[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
   for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+  for (int k = 0; k < kRing; ++k)
+    if (ctx->ring_ev[k]) (void)hipEventDestroy(ctx->ring_ev[k]);
+  if (ctx->h_ring) (void)hipHostFree(ctx->h_ring);
+  if (ctx->h_meas) (void)hipHostFree(ctx->h_meas);
+  if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
   for (void *q : ctx->allocs) (void)hipFree(q);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1239,6 +1271,7 @@ int sp_set_state(sp_ctx *ctx, double cutoff, const double *v2, const double *ss,
   }
   SP_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_flow = false;
+  ctx->flags_cached = false;
   return SP_OK;
 }
 
@@ -1286,6 +1319,7 @@ int sp_compute_step(sp_ctx *ctx) {
   SP_HIP(hipMemsetAsync(f.flags, 0, 16, ctx->stream));
   SP_HIP(launch_compute(ctx->stream, ctx->d_fd, &f, 1, ctx));
   ctx->have_flow = true;
+  ctx->flags_cached = false;
   return SP_OK;
 }
 
@@ -1306,17 +1340,20 @@ int sp_finalize_step(sp_ctx *ctx, double new_cutoff) {
   f.euler = 0;
   ctx->cutoff = new_cutoff;
   ctx->have_flow = false;
+  ctx->flags_cached = false;
   return SP_OK;
 }
 
 int sp_is_diverged(sp_ctx *ctx, int *diverged) {
   if (!ctx || !diverged) return SP_EARG;
   if (!ctx->have_flow) return fail(ctx, SP_ESTATE, "no flow computed");
-  int flags[4];
-  SP_HIP(hipMemcpyAsync(flags, ctx->fd.flags, 16, hipMemcpyDeviceToHost, ctx->stream));
-  SP_HIP(hipStreamSynchronize(ctx->stream));
-  if (flags[1]) return fail(ctx, SP_EARG, "evaluation list overflow");
-  *diverged = flags[0] ? 1 : 0;
+  if (!ctx->flags_cached) {
+    SP_HIP(hipMemcpyAsync(ctx->h_flags, ctx->fd.flags, 16, hipMemcpyDeviceToHost, ctx->stream));
+    SP_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->flags_cached = true;
+  }
+  if (ctx->h_flags[1]) return fail(ctx, SP_EARG, "evaluation list overflow");
+  *diverged = ctx->h_flags[0] ? 1 : 0;
   return SP_OK;
 }
 
@@ -1328,11 +1365,13 @@ int sp_measure_correlation(sp_ctx *ctx, double *zz, double *dd) {
   if (rc) return rc;
   SP_HIP(launch_measure(ctx->stream, ctx->d_fd, &f, 1));
   const int nout = f.B * f.Rb;
-  std::vector<double> buf(2 * (size_t)nout);
-  SP_HIP(hipMemcpyAsync(buf.data(), f.corr_out, 2 * (size_t)nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+  SP_HIP(hipMemcpyAsync(ctx->h_meas, f.corr_out, 2 * (size_t)nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (ctx->have_flow)
+    SP_HIP(hipMemcpyAsync(ctx->h_flags, f.flags, 16, hipMemcpyDeviceToHost, ctx->stream));
   SP_HIP(hipStreamSynchronize(ctx->stream));
-  std::memcpy(zz, buf.data(), (size_t)nout * 8);
-  std::memcpy(dd, buf.data() + nout, (size_t)nout * 8);
+  ctx->flags_cached = ctx->have_flow;
+  std::memcpy(zz, ctx->h_meas, (size_t)nout * 8);
+  std::memcpy(dd, ctx->h_meas + nout, (size_t)nout * 8);
   return SP_OK;
 }
 
