@@ -187,8 +187,11 @@ def run_step(ctx, prob, k, records=None):
     ctx.finalize_step(float(prob.cutoffs[k + 1]))
 
 
-def flow_leg(prob, precision, steps, warmup, label):
-    """Time `steps` iterations of the flow loop from the first cutoff."""
+def flow_leg(prob, precision, steps, warmup, label, queued=True):
+    """Time `steps` iterations of the flow loop from the first cutoff.  queued:
+    the loop runs through sp_run_flow (computeStep, takeMeasurements, isDiverged
+    and finalizeStep per step, queued without a host round trip each; records
+    read back every step); otherwise one ABI call per stage and step."""
     nsteps_flow = len(prob.cutoffs) - 1
     ctx = FlowContext(prob, precision)
     init = prob.initial_state(np.float64)
@@ -201,16 +204,31 @@ def flow_leg(prob, precision, steps, warmup, label):
     flops = 0.0
     records = []
     t0 = time.perf_counter()
-    for i in range(steps):
-        k = i % nsteps_flow
-        if k == 0 and i > 0:
-            ctx.set_state(float(prob.cutoffs[0]), *init)
-            records = []
-        run_step(ctx, prob, k, records)
-        flops += evaluation_flops(prob, prob.cutoffs[k])
-        if (i + 1) % 100 == 0:
-            print("# %s step %d / %d  cutoff %.5g  %.1f s" % (label, i + 1, steps, prob.cutoffs[k],
-                                                             time.perf_counter() - t0), file=sys.stderr, flush=True)
+    if queued:
+        done = 0
+        while done < steps:
+            n = min(steps - done, nsteps_flow)
+            if done > 0:
+                ctx.set_state(float(prob.cutoffs[0]), *init)
+            recs, diverged = ctx.run_flow(prob.cutoffs[:n + 1])
+            if diverged:
+                raise RuntimeError("flow diverged at cutoff index %d" % (len(recs) - 1))
+            records = recs
+            flops += sum(evaluation_flops(prob, prob.cutoffs[k]) for k in range(n))
+            done += n
+            print("# %s step %d / %d  %.1f s" % (label, done, steps, time.perf_counter() - t0), file=sys.stderr,
+                  flush=True)
+    else:
+        for i in range(steps):
+            k = i % nsteps_flow
+            if k == 0 and i > 0:
+                ctx.set_state(float(prob.cutoffs[0]), *init)
+                records = []
+            run_step(ctx, prob, k, records)
+            flops += evaluation_flops(prob, prob.cutoffs[k])
+            if (i + 1) % 100 == 0:
+                print("# %s step %d / %d  cutoff %.5g  %.1f s" % (label, i + 1, steps, prob.cutoffs[k],
+                                                                 time.perf_counter() - t0), file=sys.stderr, flush=True)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     dg_ms, dg_n = ctx.profile_get(KERNEL_DGAMMA)
@@ -425,7 +443,8 @@ def mode_flow(args):
         "data": "synthetic (task-file generated flow, initial vertex = model couplings)",
         "config": {"workload": args.workload, "L": prob.L, "nw": prob.nw, "n_freq_triples": prob.n_freq,
                    "overlap_terms": prob.overlap_total, "cutoffs": len(prob.cutoffs),
-                   "step": "computeStep + correlation + isDiverged + finalizeStep"},
+                   "step": "computeStep + correlation + isDiverged + finalizeStep",
+                   "loop": "sp_run_flow (steps queued, records read back every step, divergence checked on the device)"},
         "hbm_gbs_compulsory": hbm_gbs,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
@@ -436,6 +455,13 @@ def mode_flow(args):
     }
     if traffic is not None:
         out["roofline"]["traffic_gbs"] = traffic / (out["roofline"]["avg_launch_ms"] * 1e-3) / 1e9
+    if not args.no_step_api:
+        # the same flow with one ABI call per stage and step (the FrgCore call pattern)
+        rs = flow_leg(prob, SP_PRECISION_FP64, args.steps, 0, "fp64 step api", queued=False)
+        same = (len(rs["records"]) == len(r64["records"]) and
+                all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(rs["records"], r64["records"])))
+        out["step_api"] = {"evals_per_s": args.steps / rs["elapsed"], "ms_per_step": rs["elapsed"] * 1e3 / args.steps,
+                           "records_identical_to_queued": bool(same)}
     if not args.no_fixed:
         out["fixed_cutoff"] = fixed_cutoff(prob, SP_PRECISION_FP64)
     if not args.no_mixed:
@@ -547,6 +573,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-fixed", action="store_true")
+    ap.add_argument("--no-step-api", action="store_true")
     ap.add_argument("--no-tri", action="store_true")
     ap.add_argument("--no-xyz", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -106,6 +106,20 @@ int sp_is_diverged(sp_ctx *ctx, int *diverged);
  * (src/SU2/SU2MeasurementCorrelation.cpp:77-177).  Synchronous. */
 int sp_measure_correlation(sp_ctx *ctx, double *zz, double *dd);
 
+/* The cutoff loop of SpinParser::runCore (src/SpinParser.cpp:141-172) over
+ * cutoffs[0..ncut-1], the state being at cutoffs[0]: for k = 0, 1, ...
+ * computeStep at cutoffs[k], takeMeasurements (record k: B*Rb values at
+ * zz + k*B*Rb and dd + k*B*Rb), stop if isDiverged, finalizeStep(cutoffs[k+1]).
+ * The steps are queued without a host round trip each: records come back
+ * through pinned memory, a flow whose derivative holds a NaN is stopped on the
+ * device, and the host checks the records every 16 steps.  Records and final
+ * state equal those of the step-by-step calls.  *nrec = records written
+ * (ncut - 1 unless diverged), *diverged = 1 if the loop stopped on a NaN;
+ * the context is then left as the step-by-step loop leaves it (flow of the
+ * last step computed, not finalized).  zz and dd hold (ncut - 1)*B*Rb values;
+ * the final takeMeasurements of runCore is a separate sp_measure_correlation. */
+int sp_run_flow(sp_ctx *ctx, const double *cutoffs, int ncut, double *zz, double *dd, int *nrec, int *diverged);
+
 /* Wait for all work queued on the context stream. */
 int sp_synchronize(sp_ctx *ctx);
 

@@ -25,7 +25,7 @@ KERNEL_DGAMMA, KERNEL_DSIGMA, KERNEL_WEIGHTS, KERNEL_EULER = 0, 1, 2, 3
 
 EXPORTED_SYMBOLS = (
     "sp_ctx_create", "sp_ctx_destroy", "sp_set_state", "sp_get_state", "sp_get_flow", "sp_compute_step",
-    "sp_finalize_step", "sp_is_diverged", "sp_measure_correlation", "sp_synchronize", "sp_last_error",
+    "sp_finalize_step", "sp_is_diverged", "sp_measure_correlation", "sp_run_flow", "sp_synchronize", "sp_last_error",
     "sp_profile_enable", "sp_profile_get", "sp_ctx_footprint", "sp_group_create", "sp_group_destroy",
     "sp_group_step", "sp_group_measure", "sp_group_last_error",
 )
@@ -69,6 +69,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.sp_finalize_step.argtypes = [c_ctx, ctypes.c_double]
     lib.sp_is_diverged.argtypes = [c_ctx, _i32p]
     lib.sp_measure_correlation.argtypes = [c_ctx, _f64p, _f64p]
+    lib.sp_run_flow.argtypes = [c_ctx, _f64p, ctypes.c_int, _f64p, _f64p, _i32p, _i32p]
     lib.sp_synchronize.argtypes = [c_ctx]
     lib.sp_last_error.argtypes = [c_ctx]
     lib.sp_last_error.restype = ctypes.c_char_p
@@ -191,6 +192,23 @@ class FlowContext:
         dd = np.zeros(n)
         self._check(self.lib.sp_measure_correlation(self.ctx, _p(zz, _f64p), _p(dd, _f64p)), "sp_measure_correlation")
         return zz.reshape(self.prob.B, self.prob.Rb), dd.reshape(self.prob.B, self.prob.Rb)
+
+    def run_flow(self, cutoffs):
+        """The cutoff loop over ``cutoffs`` (the state at cutoffs[0]) queued by
+        ``sp_run_flow``: returns the records [(zz, dd)] of the computed steps and
+        whether the loop stopped on a diverged flow."""
+        cut = np.ascontiguousarray(cutoffs, dtype=np.float64)
+        n = self.prob.B * self.prob.Rb
+        steps = max(len(cut) - 1, 0)
+        zz = np.zeros(max(steps, 1) * n)
+        dd = np.zeros(max(steps, 1) * n)
+        nrec = ctypes.c_int(0)
+        div = ctypes.c_int(0)
+        self._check(self.lib.sp_run_flow(self.ctx, _p(cut, _f64p), len(cut), _p(zz, _f64p), _p(dd, _f64p),
+                                         ctypes.byref(nrec), ctypes.byref(div)), "sp_run_flow")
+        shape = (self.prob.B, self.prob.Rb)
+        recs = [(zz[k * n:(k + 1) * n].reshape(shape), dd[k * n:(k + 1) * n].reshape(shape)) for k in range(nrec.value)]
+        return recs, bool(div.value)
 
     def synchronize(self):
         self._check(self.lib.sp_synchronize(self.ctx), "sp_synchronize")

@@ -717,10 +717,16 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
 
 // in-place explicit Euler step (src/SU2/SU2FrgCore.cpp:111-137); a flow whose
 // derivative holds a NaN is left untouched, as the reference stops before
-// finalizeStep (src/SpinParser.cpp:151-159).  Rewrites the FP32 shadow.
+// finalizeStep (src/SpinParser.cpp:151-159), and flags[2] records that it
+// stopped: later steps enqueued behind it (sp_run_flow) leave it untouched
+// too.  Rewrites the FP32 shadow.
 __global__ void k_euler(const FlowDev *__restrict__ fl) {
   const FlowDev &F = fl[blockIdx.y];
-  if (!F.euler || F.flags[0] != 0) return;
+  if (!F.euler) return;
+  if (F.flags[0] != 0 || F.flags[2] != 0) {
+    if (F.flags[0] != 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&F.flags[2], 1);
+    return;
+  }
   const double step = F.step;
   const size_t n = (size_t)F.nv;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -860,7 +866,8 @@ __global__ void k_corr_final(const FlowDev *__restrict__ fl) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-constexpr int kRing = 8;  // pinned staging slots of the descriptor uploads
+constexpr int kRing = 8;       // pinned staging slots of the descriptor uploads
+constexpr int kRunBlock = 16;  // sp_run_flow: steps enqueued between two host checks
 
 struct sp_ctx {
   int device = 0;
@@ -877,6 +884,7 @@ struct sp_ctx {
   double *h_meas = nullptr;
   int *h_flags = nullptr;
   bool flags_cached = false;
+  double *h_run = nullptr;  // pinned records of one block of sp_run_flow steps
   double cutoff = 0.0;
   bool have_flow = false;
   size_t lds = 0;
@@ -1187,6 +1195,8 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
     ctx->h_meas = static_cast<double *>(q);
     SP_HIP(hipHostMalloc(&q, 16, 0));
     ctx->h_flags = static_cast<int *>(q);
+    SP_HIP(hipHostMalloc(&q, (size_t)kRunBlock * (2 * (size_t)f.B * f.Rb + 2) * 8, 0));
+    ctx->h_run = static_cast<double *>(q);
     for (int k = 0; k < kRing; ++k) SP_HIP(hipEventCreateWithFlags(&ctx->ring_ev[k], hipEventDisableTiming));
   }
   if (f.precision == SP_PRECISION_MIXED && (rc = dalloc(ctx, &f.G32, nv))) return rc;
@@ -1245,6 +1255,7 @@ int sp_ctx_destroy(sp_ctx *ctx) {
   if (ctx->h_ring) (void)hipHostFree(ctx->h_ring);
   if (ctx->h_meas) (void)hipHostFree(ctx->h_meas);
   if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
+  if (ctx->h_run) (void)hipHostFree(ctx->h_run);
   for (void *q : ctx->allocs) (void)hipFree(q);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1372,6 +1383,73 @@ int sp_measure_correlation(sp_ctx *ctx, double *zz, double *dd) {
   ctx->flags_cached = ctx->have_flow;
   std::memcpy(zz, ctx->h_meas, (size_t)nout * 8);
   std::memcpy(dd, ctx->h_meas + nout, (size_t)nout * 8);
+  return SP_OK;
+}
+
+int sp_run_flow(sp_ctx *ctx, const double *cutoffs, int ncut, double *zz, double *dd, int *nrec, int *diverged) {
+  if (!ctx || !cutoffs || !zz || !dd || !nrec || !diverged || ncut < 1) return fail(ctx, SP_EARG, "null argument");
+  if (!(cutoffs[0] == ctx->cutoff)) return fail(ctx, SP_EARG, "the state is not at the first cutoff");
+  for (int k = 0; k < ncut; ++k)
+    if (!std::isfinite(cutoffs[k]) || cutoffs[k] <= 0.0) return fail(ctx, SP_EARG, "cutoff must be finite and positive");
+  FlowDev &f = ctx->fd;
+  const size_t nout = (size_t)f.B * f.Rb, slot = 2 * nout + 2;
+  const int nsteps = ncut - 1;
+  *nrec = 0;
+  *diverged = 0;
+  ctx->flags_cached = false;
+  SP_HIP(hipMemsetAsync(f.flags, 0, 16, ctx->stream));  // clears the stop flag
+  for (int kb = 0; kb < nsteps; kb += kRunBlock) {
+    const int ke = nsteps - kb < kRunBlock ? nsteps : kb + kRunBlock;
+    for (int j = kb; j < ke; ++j) {
+      // computeStep and takeMeasurements at cutoffs[j]
+      f.cut = cutoffs[j];
+      f.euler = 0;
+      int rc = upload_fd(ctx);
+      if (rc) return rc;
+      SP_HIP(hipMemsetAsync(f.flags, 0, 8, ctx->stream));
+      SP_HIP(launch_compute(ctx->stream, ctx->d_fd, &f, 1, ctx));
+      SP_HIP(launch_measure(ctx->stream, ctx->d_fd, &f, 1));
+      double *rec = ctx->h_run + (size_t)(j - kb) * slot;
+      SP_HIP(hipMemcpyAsync(rec, f.corr_out, 2 * nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+      SP_HIP(hipMemcpyAsync(rec + 2 * nout, f.flags, 8, hipMemcpyDeviceToHost, ctx->stream));
+      // finalizeStep(cutoffs[j + 1]): skipped, and the flow stopped, on a NaN
+      f.step = cutoffs[j + 1] - cutoffs[j];
+      f.euler = 1;
+      f.cut = cutoffs[j + 1];
+      rc = upload_fd(ctx);
+      if (rc) return rc;
+      SP_HIP(launch_euler(ctx->stream, ctx->d_fd, &f, 1, ctx));
+      f.euler = 0;
+    }
+    SP_HIP(hipStreamSynchronize(ctx->stream));
+    for (int j = kb; j < ke; ++j) {
+      const double *rec = ctx->h_run + (size_t)(j - kb) * slot;
+      int fl[2];
+      std::memcpy(fl, rec + 2 * nout, 8);
+      if (fl[1]) return fail(ctx, SP_EARG, "evaluation list overflow");
+      std::memcpy(zz + (size_t)j * nout, rec, nout * 8);
+      std::memcpy(dd + (size_t)j * nout, rec + nout, nout * 8);
+      *nrec = j + 1;
+      if (fl[0]) {
+        // isDiverged after step j: the state stayed at cutoffs[j]; compute its
+        // flow again so that the context is the one the step-by-step loop
+        // leaves (flow of step j computed, not finalized)
+        *diverged = 1;
+        ctx->cutoff = cutoffs[j];
+        f.cut = cutoffs[j];
+        int rc = upload_fd(ctx);
+        if (rc) return rc;
+        SP_HIP(hipMemsetAsync(f.flags, 0, 16, ctx->stream));
+        SP_HIP(launch_compute(ctx->stream, ctx->d_fd, &f, 1, ctx));
+        SP_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->have_flow = true;
+        return SP_OK;
+      }
+    }
+  }
+  ctx->cutoff = cutoffs[nsteps];
+  f.cut = ctx->cutoff;
+  ctx->have_flow = false;
   return SP_OK;
 }
 
