@@ -199,6 +199,8 @@ def flow_leg(prob, precision, steps, warmup, label, queued=True):
     for w in range(warmup):
         run_step(ctx, prob, w % nsteps_flow)
     ctx.set_state(float(prob.cutoffs[0]), *init)
+    # algorithmic FLOP per evaluation, counted before the timed region
+    step_flops = [evaluation_flops(prob, prob.cutoffs[k]) for k in range(nsteps_flow)]
     ctx.synchronize()
     ctx.profile(True)
     flops = 0.0
@@ -214,7 +216,7 @@ def flow_leg(prob, precision, steps, warmup, label, queued=True):
             if diverged:
                 raise RuntimeError("flow diverged at cutoff index %d" % (len(recs) - 1))
             records = recs
-            flops += sum(evaluation_flops(prob, prob.cutoffs[k]) for k in range(n))
+            flops += sum(step_flops[:n])
             done += n
             print("# %s step %d / %d  %.1f s" % (label, done, steps, time.perf_counter() - t0), file=sys.stderr,
                   flush=True)
@@ -225,7 +227,7 @@ def flow_leg(prob, precision, steps, warmup, label, queued=True):
                 ctx.set_state(float(prob.cutoffs[0]), *init)
                 records = []
             run_step(ctx, prob, k, records)
-            flops += evaluation_flops(prob, prob.cutoffs[k])
+            flops += step_flops[k]
             if (i + 1) % 100 == 0:
                 print("# %s step %d / %d  cutoff %.5g  %.1f s" % (label, i + 1, steps, prob.cutoffs[k],
                                                                  time.perf_counter() - t0), file=sys.stderr, flush=True)
@@ -383,9 +385,9 @@ def mode_core_flow(args, core):
         ctx.measure_correlation()
         ctx.finalize_step(float(p.cutoffs[w + 1]))
     ctx.set_state(float(p.cutoffs[0]), v2, v4)
+    flops = sum(evaluation_flops(p, p.cutoffs[k]) for k in range(steps))  # counted before the timed region
     ctx.synchronize()
     ctx.profile(True)
-    flops = 0.0
     t0 = time.perf_counter()
     for k in range(steps):
         ctx.compute_step()
@@ -393,7 +395,6 @@ def mode_core_flow(args, core):
         if ctx.is_diverged():
             raise RuntimeError("%s flow diverged at cutoff index %d" % (c["label"], k))
         ctx.finalize_step(float(p.cutoffs[k + 1]))
-        flops += evaluation_flops(p, p.cutoffs[k])
         if (k + 1) % 20 == 0:
             print("# %s step %d / %d  %.1f s" % (core, k + 1, steps, time.perf_counter() - t0), file=sys.stderr,
                   flush=True)

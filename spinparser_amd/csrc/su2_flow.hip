@@ -859,6 +859,13 @@ __global__ void k_corr_final(const FlowDev *__restrict__ fl) {
     F.corr_out[k] = susc[2 * F.corr_rid[k]];
     F.corr_out[nout + k] = susc[2 * F.corr_rid[k] + 1];
   }
+  if (threadIdx.x == 0) {
+    // the flags of the step after the record, so that one copy returns both
+    const int2 fl = make_int2(F.flags[0], F.flags[1]);
+    double packed;
+    __builtin_memcpy(&packed, &fl, 8);
+    F.corr_out[2 * nout] = packed;
+  }
 }
 
 }  // namespace
@@ -884,6 +891,7 @@ struct sp_ctx {
   double *h_meas = nullptr;
   int *h_flags = nullptr;
   bool flags_cached = false;
+  double list_cut = -1.0;   // cutoff of the correlation point list on the device
   double *h_run = nullptr;  // pinned records of one block of sp_run_flow steps
   double cutoff = 0.0;
   bool have_flow = false;
@@ -1091,9 +1099,11 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
   return hipSuccess;
 }
 
-hipError_t launch_measure(hipStream_t st, const FlowDev *d, const FlowDev *h, int n) {
+// build_list: rebuild the correlation point list; a dGamma launch at the same
+// cutoff (k_weights over x <= nw) has built it already otherwise
+hipError_t launch_measure(hipStream_t st, const FlowDev *d, const FlowDev *h, int n, bool build_list) {
   const Geometry geo = geometry(h, n);
-  hipLaunchKernelGGL(k_weights, dim3(1, n), dim3(64), geo.maxnw * sizeof(double), st, d, 1);
+  if (build_list) hipLaunchKernelGGL(k_weights, dim3(1, n), dim3(64), geo.maxnw * sizeof(double), st, d, 1);
   hipLaunchKernelGGL(k_corr_partial, dim3(geo.maxemax, n), dim3(kThreads), 0, st, d);
   hipLaunchKernelGGL(k_corr_final, dim3(1, n), dim3(256), 0, st, d);
   return hipGetLastError();
@@ -1184,14 +1194,14 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
       (rc = dalloc(ctx, &f.Wc, (size_t)f.nw * emax)) || (rc = dalloc(ctx, &f.Wn, f.nw)) ||
       (rc = dalloc(ctx, &f.Cwp, emax)) || (rc = dalloc(ctx, &f.Cw, emax)) || (rc = dalloc(ctx, &f.Cn, 1)) ||
       (rc = dalloc(ctx, &f.corr_partial, (size_t)emax * f.L * 2)) ||
-      (rc = dalloc(ctx, &f.corr_out, (size_t)2 * f.B * f.Rb)) || (rc = dalloc(ctx, &f.flags, 4)) ||
+      (rc = dalloc(ctx, &f.corr_out, (size_t)2 * f.B * f.Rb + 1)) || (rc = dalloc(ctx, &f.flags, 4)) ||
       (rc = dalloc(ctx, &ctx->d_fd, 1)))
     return rc;
   {
     void *q = nullptr;
     SP_HIP(hipHostMalloc(&q, kRing * sizeof(FlowDev), 0));
     ctx->h_ring = static_cast<FlowDev *>(q);
-    SP_HIP(hipHostMalloc(&q, (size_t)2 * f.B * f.Rb * 8, 0));
+    SP_HIP(hipHostMalloc(&q, ((size_t)2 * f.B * f.Rb + 1) * 8, 0));
     ctx->h_meas = static_cast<double *>(q);
     SP_HIP(hipHostMalloc(&q, 16, 0));
     ctx->h_flags = static_cast<int *>(q);
@@ -1331,6 +1341,7 @@ int sp_compute_step(sp_ctx *ctx) {
   SP_HIP(launch_compute(ctx->stream, ctx->d_fd, &f, 1, ctx));
   ctx->have_flow = true;
   ctx->flags_cached = false;
+  ctx->list_cut = f.cut;
   return SP_OK;
 }
 
@@ -1374,12 +1385,13 @@ int sp_measure_correlation(sp_ctx *ctx, double *zz, double *dd) {
   f.cut = ctx->cutoff;
   int rc = upload_fd(ctx);
   if (rc) return rc;
-  SP_HIP(launch_measure(ctx->stream, ctx->d_fd, &f, 1));
+  SP_HIP(launch_measure(ctx->stream, ctx->d_fd, &f, 1, !(ctx->list_cut == f.cut)));
+  ctx->list_cut = f.cut;
   const int nout = f.B * f.Rb;
-  SP_HIP(hipMemcpyAsync(ctx->h_meas, f.corr_out, 2 * (size_t)nout * 8, hipMemcpyDeviceToHost, ctx->stream));
-  if (ctx->have_flow)
-    SP_HIP(hipMemcpyAsync(ctx->h_flags, f.flags, 16, hipMemcpyDeviceToHost, ctx->stream));
+  SP_HIP(hipMemcpyAsync(ctx->h_meas, f.corr_out, (2 * (size_t)nout + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
   SP_HIP(hipStreamSynchronize(ctx->stream));
+  // the flags that k_corr_final stored after the record
+  std::memcpy(ctx->h_flags, ctx->h_meas + 2 * nout, 8);
   ctx->flags_cached = ctx->have_flow;
   std::memcpy(zz, ctx->h_meas, (size_t)nout * 8);
   std::memcpy(dd, ctx->h_meas + nout, (size_t)nout * 8);
@@ -1408,10 +1420,9 @@ int sp_run_flow(sp_ctx *ctx, const double *cutoffs, int ncut, double *zz, double
       if (rc) return rc;
       SP_HIP(hipMemsetAsync(f.flags, 0, 8, ctx->stream));
       SP_HIP(launch_compute(ctx->stream, ctx->d_fd, &f, 1, ctx));
-      SP_HIP(launch_measure(ctx->stream, ctx->d_fd, &f, 1));
-      double *rec = ctx->h_run + (size_t)(j - kb) * slot;
-      SP_HIP(hipMemcpyAsync(rec, f.corr_out, 2 * nout * 8, hipMemcpyDeviceToHost, ctx->stream));
-      SP_HIP(hipMemcpyAsync(rec + 2 * nout, f.flags, 8, hipMemcpyDeviceToHost, ctx->stream));
+      SP_HIP(launch_measure(ctx->stream, ctx->d_fd, &f, 1, false));
+      double *rec = ctx->h_run + (size_t)(j - kb) * slot;  // record, then the step's flags
+      SP_HIP(hipMemcpyAsync(rec, f.corr_out, (2 * nout + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
       // finalizeStep(cutoffs[j + 1]): skipped, and the flow stopped, on a NaN
       f.step = cutoffs[j + 1] - cutoffs[j];
       f.euler = 1;
@@ -1443,6 +1454,7 @@ int sp_run_flow(sp_ctx *ctx, const double *cutoffs, int ncut, double *zz, double
         SP_HIP(launch_compute(ctx->stream, ctx->d_fd, &f, 1, ctx));
         SP_HIP(hipStreamSynchronize(ctx->stream));
         ctx->have_flow = true;
+        ctx->list_cut = cutoffs[j];
         return SP_OK;
       }
     }
@@ -1450,6 +1462,7 @@ int sp_run_flow(sp_ctx *ctx, const double *cutoffs, int ncut, double *zz, double
   ctx->cutoff = cutoffs[nsteps];
   f.cut = ctx->cutoff;
   ctx->have_flow = false;
+  ctx->list_cut = nsteps > 0 ? cutoffs[nsteps - 1] : ctx->list_cut;
   return SP_OK;
 }
 
@@ -1545,6 +1558,7 @@ static int group_prepare(sp_group *grp, sp_ctx *const *ctxs, int n, size_t *out_
   for (int j = 0; j < n; ++j) {
     sp_ctx *c = ctxs[order[j]];
     SP_GHIP(hipStreamSynchronize(c->stream));
+    c->list_cut = -1.0;  // the group's launches rebuild the flow's point lists
     FlowDev f = c->fd;
     if (launch_class(f) != prev) { base = 0; prev = launch_class(f); }
     f.item_base = base;
@@ -1571,14 +1585,8 @@ int sp_group_step(sp_group *grp, sp_ctx *const *ctxs, int n, const double *new_c
   }
   SP_GHIP(hipMemcpyAsync(grp->d_fd, grp->h_fd.data(), sizeof(FlowDev) * n, hipMemcpyHostToDevice, grp->stream));
   SP_GHIP(launch_compute(grp->stream, grp->d_fd, grp->h_fd.data(), n, nullptr));
-  SP_GHIP(launch_measure(grp->stream, grp->d_fd, grp->h_fd.data(), n));
-  // divergence flags next to each flow's correlation record
-  for (int j = 0; j < n; ++j) {
-    const int i = grp->h_fd[j].pad_index;
-    const FlowDev &f = grp->h_fd[j];
-    SP_GHIP(hipMemcpyAsync(grp->d_out + off[i] + 2 * (size_t)f.B * f.Rb, f.flags, 8, hipMemcpyDeviceToDevice,
-                           grp->stream));
-  }
+  // the divergence flags land next to each flow's record (k_corr_final)
+  SP_GHIP(launch_measure(grp->stream, grp->d_fd, grp->h_fd.data(), n, false));
   if (new_cutoffs) {
     for (int j = 0; j < n; ++j) {
       const int i = grp->h_fd[j].pad_index;
@@ -1622,7 +1630,7 @@ int sp_group_measure(sp_group *grp, sp_ctx *const *ctxs, int n, double *const *z
   int rc = group_prepare(grp, ctxs, n, off.data());
   if (rc) return rc;
   SP_GHIP(hipMemcpyAsync(grp->d_fd, grp->h_fd.data(), sizeof(FlowDev) * n, hipMemcpyHostToDevice, grp->stream));
-  SP_GHIP(launch_measure(grp->stream, grp->d_fd, grp->h_fd.data(), n));
+  SP_GHIP(launch_measure(grp->stream, grp->d_fd, grp->h_fd.data(), n, true));
   size_t total = 0;
   for (int i = 0; i < n; ++i) total = std::max(total, off[i] + 2 * (size_t)ctxs[i]->fd.B * ctxs[i]->fd.Rb);
   SP_GHIP(hipMemcpyAsync(grp->h_out, grp->d_out, total * 8, hipMemcpyDeviceToHost, grp->stream));
