@@ -131,7 +131,7 @@ __device__ __forceinline__ double2 direct2(const double2 *G, int L, int nw, int 
 // SU2VertexTwoParticle::getValue(..., FrequencyChannel::None) for both channels.
 // rid_fwd / rid_inv: symmetryTransform(i1, i2) and symmetryTransform(i2, i1).
 __device__ double2 value_none(const double2 *G, const double *m, int nw, int L, int rid_fwd, int rid_inv, double s,
-                              double t, double u) {
+                              double t, double u, int geo = 0, float ga = 0.0f, float gb = 0.0f) {
   bool swap = false;
   if (s < 0 && u < 0) { s = -s; u = -u; }
   else {
@@ -142,9 +142,9 @@ __device__ double2 value_none(const double2 *G, const double *m, int nw, int L, 
   const int rid = swap ? rid_inv : rid_fwd;
   int lS, uS, lT, uT, lU, uU;
   double bS, bT, bU;
-  interp(m, nw, s, lS, uS, bS);
-  interp(m, nw, t, lT, uT, bT);
-  interp(m, nw, u, lU, uU, bU);
+  interp_sel(m, nw, s, geo, ga, gb, lS, uS, bS);
+  interp_sel(m, nw, t, geo, ga, gb, lT, uT, bT);
+  interp_sel(m, nw, u, geo, ga, gb, lU, uU, bU);
   double2 r;
   {
     double2 a000 = direct2(G, L, nw, rid, lS, lT, lU), a100 = direct2(G, L, nw, rid, uS, lT, lU);
@@ -208,30 +208,39 @@ __global__ __launch_bounds__(64) void k_weights(const FlowDev *__restrict__ fl, 
 }
 
 // ---------------------------------------------------------------------------
-// dSigma/dLambda (src/SU2/SU2FrgCore.cpp:139-169): one wavefront per mesh point
+// dSigma/dLambda (src/SU2/SU2FrgCore.cpp:139-169): one 256-thread workgroup
+// per mesh point, the range sum spread over its four waves; the mesh in LDS
+// (dynamic, nw doubles)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_dsigma(const FlowDev *__restrict__ fl) {
+__global__ __launch_bounds__(256) void k_dsigma(const FlowDev *__restrict__ fl) {
+  extern __shared__ double s_dmesh[];
+  __shared__ double part[4];
   const FlowDev &F = fl[blockIdx.y];
   const int i = blockIdx.x;
   if (i >= F.nw) return;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
   const int nw = F.nw, L = F.L;
-  const double *m = F.mesh;
+  for (int k = tid; k < nw; k += 256) s_dmesh[k] = F.mesh[k];
+  __syncthreads();
+  const double *m = s_dmesh;
   const double2 *G = F.G;
   const double cut = F.cut;
   const double w = m[i];
   double sum = 0.0;
-  for (int j = lane; j < F.R; j += 64) {
-    const double2 a = value_none(G, m, nw, L, F.range_fwd[j], F.range_inv[j], w + cut, 0.0, w - cut);
-    const double2 b = value_none(G, m, nw, L, F.range_fwd[j], F.range_inv[j], w - cut, 0.0, w + cut);
+  for (int j = tid; j < F.R; j += 256) {
+    const double2 a = value_none(G, m, nw, L, F.range_fwd[j], F.range_inv[j], w + cut, 0.0, w - cut, F.geo, F.ga, F.gb);
+    const double2 b = value_none(G, m, nw, L, F.range_fwd[j], F.range_inv[j], w - cut, 0.0, w + cut, F.geo, F.ga, F.gb);
     sum += a.y - b.y;
   }
   sum = wave_sum(sum);
-  if (lane == 0) {
+  if (lane == 0) part[tid >> 6] = sum;
+  __syncthreads();
+  if (tid == 0) {
+    sum = (part[0] + part[1]) + (part[2] + part[3]);
     double v = 0.0;
     v -= 4.0 * F.spin * sum;
-    const double2 p = value_none(G, m, nw, L, 0, 0, w + cut, w - cut, 0.0);
-    const double2 q = value_none(G, m, nw, L, 0, 0, w - cut, w + cut, 0.0);
+    const double2 p = value_none(G, m, nw, L, 0, 0, w + cut, w - cut, 0.0, F.geo, F.ga, F.gb);
+    const double2 q = value_none(G, m, nw, L, 0, 0, w - cut, w + cut, 0.0, F.geo, F.ga, F.gb);
     v += 0.75 * (p.x - q.x);
     v += (p.y - q.y);
     v /= (kTwoPi * (cut + sigma_at(m, nw, F.v2, cut)));
@@ -760,12 +769,15 @@ __global__ void k_shadow(const FlowDev *__restrict__ fl) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_corr_partial(const FlowDev *__restrict__ fl) {
   __shared__ double red[kThreads * 2];
+  extern __shared__ double s_cmesh[];  // the mesh (dynamic, nw doubles)
   const FlowDev &F = fl[blockIdx.y];
   const int o = blockIdx.x;
   const int n = *F.Cn;
   if (o >= n) return;
   const int nw = F.nw, L = F.L;
-  const double *m = F.mesh, *v2 = F.v2;
+  for (int k = threadIdx.x; k < nw; k += kThreads) s_cmesh[k] = F.mesh[k];
+  __syncthreads();
+  const double *m = s_cmesh, *v2 = F.v2;
   const double2 *G = F.G;
   const double spin = F.spin;
   const int slots = kThreads / L;
@@ -790,9 +802,9 @@ __global__ __launch_bounds__(kThreads) void k_corr_partial(const FlowDev *__rest
       if (b < 0) b = -b;
       int lS, uS, lT, uT, lU, uU;
       double bS, bT, bU;
-      interp(m, nw, a, lS, uS, bS);
-      interp(m, nw, b, lT, uT, bT);
-      interp(m, nw, c, lU, uU, bU);
+      interp_sel(m, nw, a, F.geo, F.ga, F.gb, lS, uS, bS);
+      interp_sel(m, nw, b, F.geo, F.ga, F.gb, lT, uT, bT);
+      interp_sel(m, nw, c, F.geo, F.ga, F.gb, lU, uU, bU);
       const int sr = exch ? F.inv_rid[rid] : F.site_rid[rid];
       const int S3[8] = {lS, uS, lS, uS, lS, uS, lS, uS};
       const int T3[8] = {lT, lT, uT, uT, lT, lT, uT, uT};
@@ -812,7 +824,7 @@ __global__ __launch_bounds__(kThreads) void k_corr_partial(const FlowDev *__rest
       double rs = -(spin * spin) * sbs;
       double rd = -(16.0 * spin * spin) * sbd;
       if (rid == 0) {
-        const double2 vv = value_none(G, m, nw, L, 0, 0, w + wp + nu, w - wp, nu);
+        const double2 vv = value_none(G, m, nw, L, 0, 0, w + wp + nu, w - wp, nu, F.geo, F.ga, F.gb);
         rs += spin * (-vv.x / 4.0 + vv.y);
         rd += spin * (3.0 * vv.x + 4.0 * vv.y);
       }
@@ -1062,7 +1074,7 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
   const Geometry geo = geometry(h, n);
   {
     ProfScope ps(prof, 1);
-    hipLaunchKernelGGL(k_dsigma, dim3(geo.maxnw, n), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_dsigma, dim3(geo.maxnw, n), dim3(256), geo.maxnw * sizeof(double), st, d);
   }
   {
     ProfScope ps(prof, 2);
@@ -1104,7 +1116,7 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
 hipError_t launch_measure(hipStream_t st, const FlowDev *d, const FlowDev *h, int n, bool build_list) {
   const Geometry geo = geometry(h, n);
   if (build_list) hipLaunchKernelGGL(k_weights, dim3(1, n), dim3(64), geo.maxnw * sizeof(double), st, d, 1);
-  hipLaunchKernelGGL(k_corr_partial, dim3(geo.maxemax, n), dim3(kThreads), 0, st, d);
+  hipLaunchKernelGGL(k_corr_partial, dim3(geo.maxemax, n), dim3(kThreads), geo.maxnw * sizeof(double), st, d);
   hipLaunchKernelGGL(k_corr_final, dim3(1, n), dim3(256), 0, st, d);
   return hipGetLastError();
 }
