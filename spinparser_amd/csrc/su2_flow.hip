@@ -401,7 +401,82 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
         const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
         const int n = ch == 0 ? nS : (ch == 1 ? nT : nU);
         const int nvalid = n - e0 < nb ? n - e0 : nb;
-        if (lane < 2 * nvalid) {
+        if (!MIXED) {
+          // FP64 form: four tasks per point, one access buffer of a pair each (in the t
+          // channel one gathered and one site-local buffer), the buffer's
+          // arguments picked arithmetically so that every lane runs the same
+          // straight-line code: the wave issues one descriptor assembly per
+          // lane instead of looping over the pair's buffers
+          const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
+          const double w1p = 0.5 * (s + t + u);
+          const double w1 = 0.5 * (s - t + u);
+          const double w2p = 0.5 * (s - t - u);
+          const double w2 = 0.5 * (s + t - u);
+          double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
+          int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
+          V2 *s_w = reinterpret_cast<V2 *>(lds_raw + lay.w + bf * lay.dbuf);
+          V2 *s_wd = reinterpret_cast<V2 *>(lds_raw + lay.wd + bf * lay.dbuf);
+          int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex + bf * lay.dbuf);
+          V *s_loc = reinterpret_cast<V *>(lds_raw + lay.loc + bf * lay.dbuf);
+          for (int task = lane; task < 4 * nvalid; task += 64) {
+            const int e = task >> 2, pr = (task >> 1) & 1, h = task & 1;
+            const double wp = F.Wwp[(size_t)xo * F.emax + e0 + e];
+            // the (point, pair) buffers of SU2FrgCore.cpp:199-334 (k = access
+            // buffer, a / c = signed s / u arguments, swap: interpolations fed
+            // as (slot 2, slot 1)); buffer h of the pair
+            double x0, x1;
+            int exb;  // exactly one of the signed s / u arguments negative: site exchange
+            if (ch == 0) {
+              x0 = fabs(pr ? w1p + wp : w1 + wp);
+              x1 = fabs(pr ? w2p + wp : w2 + wp);
+              const double c = h ? (pr ? w1p + wp : w1 + wp) : (pr ? -w2p - wp : -w2 - wp);
+              exb = (s < 0) != (c < 0);
+            } else if (ch == 1) {
+              const double A = pr ? w2p - wp : w1 - wp, B = pr ? w2 + wp : w1p + wp;
+              x0 = fabs(A);
+              x1 = fabs(B);
+              const double c = h ? (pr ? wp - w2p : w1 - wp) : (pr ? -w2 - wp : w1p + wp);
+              exb = ((h ? B : A) < 0) != (c < 0);
+            } else {
+              x0 = fabs(pr ? w1p + wp : w1 + wp);
+              x1 = fabs(pr ? w2 - wp : w2p - wp);
+              const double a = h ? (pr ? w2 - wp : w2p - wp) : (pr ? w1p + wp : w1 + wp);
+              exb = (a < 0) != (u < 0);
+            }
+            // swap: the buffer takes the pair's interpolations in the other order
+            const double xa = h ? x1 : x0, xb = h ? x0 : x1;
+            Interp ia, ib;
+            if (F.geo) interp_pair_geo(s_mesh, nw, xa, xb, F.ga, F.gb, ia, ib);
+            else interp_pair(s_mesh, nw, xa, xb, ia, ib);
+            {
+              const Desc4 d = assemble(ch, xo, ia, ib, nw, rowbytes);
+              const int sk = e * 4 + 2 * h + pr;
+              s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
+              s_w[2 * sk] = mk2<V2>(V(d.ws[0]), V(d.ws[1]));
+              s_w[2 * sk + 1] = mk2<V2>(V(d.ws[2]), V(d.ws[3]));
+              s_wd[2 * sk] = mk2<V2>(V(d.wd[0]), V(d.wd[1]));
+              s_wd[2 * sk + 1] = mk2<V2>(V(d.wd[2]), V(d.wd[3]));
+              s_ex[sk] = exb;
+            }
+            if (ch == 1) {
+              // site-local buffer (k = 5, 7 for pair 0; 4, 6 for pair 1) with
+              // the same interpolations: SU2VertexTwoParticle::getValueLocal
+              // (:347-360), site 0, no exchange
+              const Desc4 d = assemble(2, xo, ia, ib, nw, rowbytes);
+              V vs = V(0), vd = V(0);
+  #pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const V2 v = buf_load<V2>(rG, d.off[i]);
+                vs = fma(V(d.ws[i]), v.x, vs);
+                vd = fma(V(d.wd[i]), v.y, vd);
+              }
+              const int kl = (pr ? 0 : 1) + 2 * h;  // local index k - 4
+              s_loc[(e * 4 + kl) * 2 + 0] = fma(V(0.25), vs, -vd);
+              s_loc[(e * 4 + kl) * 2 + 1] = -fma(V(0.75), vs, vd);
+            }
+            if (pr == 0 && h == 0) s_c[e] = (ch == 2 ? -1.0 : 1.0) * F.Wc[(size_t)xo * F.emax + e0 + e];
+          }
+        } else if (lane < 2 * nvalid) {
           // transfer frequencies (SU2FrgCore.cpp:193-197), recomputed here so
           // that they occupy no registers in the worker code
           const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
