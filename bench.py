@@ -106,11 +106,11 @@ CORES = {
 # 2.00; WRITE_SIZE reads them exactly, scale 1.00).  The profile summaries
 # under profiles/ hold the runs and the commands.
 TRAFFIC_KIB = {
-    ("cubic-j1j2-r6-nw64", "f64"): {"fetch_kib": 742076.0, "write_kib": 47864.5,
+    ("cubic-j1j2-r6-nw64", "f64"): {"fetch_kib": 740179.0, "write_kib": 47865.1,
                                     "fetch_scale": 2.00, "write_scale": 1.00},
     ("kagome-dm-r5-nw48", "f64"): {"fetch_kib": 21931400.0, "write_kib": 134064.0,
                                    "fetch_scale": 2.00, "write_scale": 1.00},
-    ("honeycomb-kitaev-r8-nw64", "f64"): {"fetch_kib": 3351270.0, "write_kib": 96780.7,
+    ("honeycomb-kitaev-r8-nw64", "f64"): {"fetch_kib": 3350080.0, "write_kib": 96774.9,
                                           "fetch_scale": 2.00, "write_scale": 1.00},
 }
 

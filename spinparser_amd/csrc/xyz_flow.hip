@@ -318,101 +318,86 @@ __global__ __launch_bounds__(kXyzPW, SPFRG_XYZ_MIN_BLOCKS) void k_xyz_dgamma(Xyz
         const int xo = ch == 0 ? so : (ch == 1 ? to : uo);
         const int n = ch == 0 ? nS : (ch == 1 ? nT : nU);
         const int nvalid = n - e0 < nb ? n - e0 : nb;
-        // (A) one lane per (point, pair): the two bracket searches shared by
-        // the 2 (s, u channels) or 4 (t channel) access buffers of the pair
-        // (XYZFrgCore.cpp:226-230, 281-287, 325-331, 423-427)
-        if (lane < 2 * nvalid) {
-          const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
-          const double w1p = 0.5 * (s + t + u);
-          const double w1 = 0.5 * (s - t + u);
-          const double w2p = 0.5 * (s - t - u);
-          const double w2 = 0.5 * (s + t - u);
-          double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
-          int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
-          double *s_ws = reinterpret_cast<double *>(lds_raw + lay.ws + bf * lay.dbuf);
-          int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex + bf * lay.dbuf);
-          double *s_loc = reinterpret_cast<double *>(lds_raw + lay.loc + bf * lay.dbuf);
-          const int e = lane >> 1, pr = lane & 1;
+        // (A) four tasks per point, one access buffer of a (point, pair) each
+        // (in the t channel one gathered and one site-local buffer), with the
+        // buffer's arguments picked arithmetically so that every lane runs the
+        // same straight-line code (XYZFrgCore.cpp:226-230, 281-287, 325-331,
+        // 423-427)
+        const double s = s_mesh[so], t = s_mesh[to], u = s_mesh[uo];
+        const double w1p = 0.5 * (s + t + u);
+        const double w1 = 0.5 * (s - t + u);
+        const double w2p = 0.5 * (s - t - u);
+        const double w2 = 0.5 * (s + t - u);
+        double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
+        int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
+        double *s_ws = reinterpret_cast<double *>(lds_raw + lay.ws + bf * lay.dbuf);
+        int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex + bf * lay.dbuf);
+        double *s_loc = reinterpret_cast<double *>(lds_raw + lay.loc + bf * lay.dbuf);
+        for (int task = lane; task < 4 * nvalid; task += 64) {
+          const int e = task >> 2, pr = (task >> 1) & 1, h = task & 1;
           const double wp = F.Wwp[(size_t)xo * F.emax + e0 + e];
           const double c = F.Wc[(size_t)xo * F.emax + e0 + e];
+          // buffer h of pair pr: |bracket arguments| x0 / x1 of the pair, and
+          // whether exactly one of the buffer's signed s / u arguments is
+          // negative (site exchange); buffer 1 takes the interpolations in
+          // the other order
           double x0, x1;
-          PairBuf pb[4];
-          int nbufs = 2;
+          int exb;
           if (ch == 0) {
-            if (pr == 0) {
-              x0 = fabs(w1 + wp); x1 = fabs(w2 + wp);
-              pb[0] = {0, s, -w2 - wp, false, 0};
-              pb[1] = {2, s, w1 + wp, true, 0};
-            } else {
-              x0 = fabs(w1p + wp); x1 = fabs(w2p + wp);
-              pb[0] = {1, s, -w2p - wp, false, 0};
-              pb[1] = {3, s, w1p + wp, true, 0};
-            }
+            x0 = fabs(pr ? w1p + wp : w1 + wp);
+            x1 = fabs(pr ? w2p + wp : w2 + wp);
+            const double cc = h ? (pr ? w1p + wp : w1 + wp) : (pr ? -w2p - wp : -w2 - wp);
+            exb = (s < 0) != (cc < 0);
           } else if (ch == 1) {
-            nbufs = 4;
-            if (pr == 0) {
-              x0 = fabs(w1 - wp); x1 = fabs(w1p + wp);
-              pb[0] = {0, w1 - wp, w1p + wp, false, 1};
-              pb[1] = {2, w1p + wp, w1 - wp, true, 1};
-              pb[2] = {5, w1 - wp, -t, false, 2};
-              pb[3] = {7, w1p + wp, -t, true, 2};
-            } else {
-              x0 = fabs(w2p - wp); x1 = fabs(w2 + wp);
-              pb[0] = {1, w2p - wp, -w2 - wp, false, 1};
-              pb[1] = {3, w2 + wp, wp - w2p, true, 1};
-              pb[2] = {4, w2p - wp, t, false, 2};
-              pb[3] = {6, w2 + wp, t, true, 2};
-            }
+            const double A = pr ? w2p - wp : w1 - wp, B = pr ? w2 + wp : w1p + wp;
+            x0 = fabs(A);
+            x1 = fabs(B);
+            const double cc = h ? (pr ? wp - w2p : w1 - wp) : (pr ? -w2 - wp : w1p + wp);
+            exb = ((h ? B : A) < 0) != (cc < 0);
           } else {
-            if (pr == 0) {
-              x0 = fabs(w1 + wp); x1 = fabs(w2p - wp);
-              pb[0] = {0, w1 + wp, u, false, 2};
-              pb[1] = {2, w2p - wp, u, true, 2};
-            } else {
-              x0 = fabs(w1p + wp); x1 = fabs(w2 - wp);
-              pb[0] = {1, w1p + wp, u, false, 2};
-              pb[1] = {3, w2 - wp, u, true, 2};
-            }
+            x0 = fabs(pr ? w1p + wp : w1 + wp);
+            x1 = fabs(pr ? w2 - wp : w2p - wp);
+            const double a = h ? (pr ? w2 - wp : w2p - wp) : (pr ? w1p + wp : w1 + wp);
+            exb = (a < 0) != (u < 0);
           }
-          Interp i0, i1;
-          if (F.geo) interp_pair_geo(s_mesh, nw, x0, x1, F.ga, F.gb, i0, i1);
-          else interp_pair(s_mesh, nw, x0, x1, i0, i1);
-          // fully unrolled with a guard so that pb[] stays in registers
+          const double xa = h ? x1 : x0, xb = h ? x0 : x1;
+          Interp ia, ib;
+          if (F.geo) interp_pair_geo(s_mesh, nw, xa, xb, F.ga, F.gb, ia, ib);
+          else interp_pair(s_mesh, nw, xa, xb, ia, ib);
+          {
+            const Desc4 d = assemble(ch, xo, ia, ib, nw, rowbytes);
+            const int sk = e * 4 + 2 * h + pr;
+            s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            if (q >= nbufs) break;
-            const PairBuf &bb = pb[q];
-            const Desc4 d = assemble(bb.chan, xo, bb.swap ? i1 : i0, bb.swap ? i0 : i1, nw, rowbytes);
-            if (bb.k < 4) {
-              const int sk = e * 4 + bb.k;
-              s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
-#pragma unroll
-              for (int r = 0; r < 4; ++r) s_ws[sk * 4 + r] = d.ws[r];
-              // bit 0: site exchange; bits 1..4: density weight of support r negated
-              s_ex[sk] = (((bb.a < 0) != (bb.c < 0)) ? 1 : 0) | (d.neg << 1);
-            } else {
-              // getValueLocal (XYZVertexTwoParticle.hpp:355-376): site 0, no exchange
-              double l[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-              for (int r = 0; r < 4; ++r) {
-                const double2 v01 = buf_load<double2>(rG, d.off[r]);
-                const double2 v23 = buf_load<double2>(rG, d.off[r] + 16);
-                l[kX] = fma(d.ws[r], v01.x, l[kX]);
-                l[kY] = fma(d.ws[r], v01.y, l[kY]);
-                l[kZ] = fma(d.ws[r], v23.x, l[kZ]);
-                l[kD] = fma(d.wd[r], v23.y, l[kD]);
-              }
-              // chalice / inverse chalice factors of this local vertex, scaled
-              // by the point's weight (XYZFrgCore.cpp:333-416): spin output k
-              // gets the two other spin components with +, the component
-              // itself and the density with -; the density output -sum
-              double *fo = s_loc + e * 16 + (bb.k - 4) * 4;
-#pragma unroll
-              for (int k = 0; k < 3; ++k) fo[k] = c * chal(l, k);
-              fo[kD] = -(c * ((l[kD] + l[kX]) + (l[kY] + l[kZ])));
-            }
+            for (int r = 0; r < 4; ++r) s_ws[sk * 4 + r] = d.ws[r];
+            // bit 0: site exchange; bits 1..4: density weight of support r negated
+            s_ex[sk] = exb | (d.neg << 1);
           }
-          if (pr == 0) s_c[e] = c;
+          if (ch == 1) {
+            // site-local buffer (k = 5, 7 for pair 0; 4, 6 for pair 1) with
+            // the same interpolations: getValueLocal
+            // (XYZVertexTwoParticle.hpp:355-376), site 0, no exchange
+            const Desc4 d = assemble(2, xo, ia, ib, nw, rowbytes);
+            double l[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const double2 v01 = buf_load<double2>(rG, d.off[r]);
+              const double2 v23 = buf_load<double2>(rG, d.off[r] + 16);
+              l[kX] = fma(d.ws[r], v01.x, l[kX]);
+              l[kY] = fma(d.ws[r], v01.y, l[kY]);
+              l[kZ] = fma(d.ws[r], v23.x, l[kZ]);
+              l[kD] = fma(d.wd[r], v23.y, l[kD]);
+            }
+            // chalice / inverse chalice factors of this local vertex, scaled
+            // by the point's weight (XYZFrgCore.cpp:333-416): spin output k
+            // gets the two other spin components with +, the component
+            // itself and the density with -; the density output -sum
+            double *fo = s_loc + e * 16 + ((pr ? 0 : 1) + 2 * h) * 4;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) fo[k] = c * chal(l, k);
+            fo[kD] = -(c * ((l[kD] + l[kX]) + (l[kY] + l[kZ])));
+          }
+          if (pr == 0 && h == 0) s_c[e] = c;
         }
       }
     } else if (active && jc >= 0) {
