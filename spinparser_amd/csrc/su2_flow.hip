@@ -31,12 +31,12 @@
 //                  (buffer loads with 32-bit offsets) and the channel algebra
 //                  from the other descriptor buffer;
 //              (C, t channel) the RPA lattice bubble accumulated as
-//                  C[a][b] += sum_e c_e A2_e[a] A3_e[b] in LDS (2x2 register
-//                  tiles, all waves), contracted with the overlap lists once
-//                  per item.
+//                  C[a][b] += sum_e c_e A2_e[a] A3_e[b] in FP64 in LDS (2x4
+//                  register tiles, 2x2 in the mixed form, all waves),
+//                  contracted with the overlap lists once per item.
 //              Template MIXED: FP32 gathers and algebra with compensated
-//              (two-sum) accumulation; descriptors, frequencies and the result
-//              stay FP64.
+//              (two-sum) accumulation; the lattice-bubble sums, descriptors,
+//              frequencies and the result stay FP64.
 //   k_euler    in-place Gamma += dLambda * dGamma (streaming, HBM bound);
 //              skipped for a flow whose derivative holds a NaN.
 //   k_corr_*   static susceptibility in FP64, one workgroup per (flow, outer
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
   double *s_mesh = reinterpret_cast<double *>(lds_raw + lay.mesh);
   V *s_P = reinterpret_cast<V *>(lds_raw + lay.P);
   V *s_Q = reinterpret_cast<V *>(lds_raw + lay.Q);
-  double *s_C = reinterpret_cast<double *>(lds_raw + lay.C);  // FP64 sums, or {sum, comp} float pairs
+  double *s_C = reinterpret_cast<double *>(lds_raw + lay.C);  // FP64 lattice-bubble sums (both forms)
   double *s_rpa = reinterpret_cast<double *>(lds_raw + lay.rpa);
   double *s_red = reinterpret_cast<double *>(lds_raw + lay.red);
   int *s_site = reinterpret_cast<int *>(lds_raw + lay.site);
@@ -684,8 +684,10 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
     }
     __syncthreads();
     if (jc >= 0 && ch == 1) {
-      // (C) C[ch][a][b] += sum_e P[ch][e][a] Q[ch][e][b]: FP64 in TR x TC
-      // register tiles, the compensated FP32 sums of the mixed form in 2x2
+      // (C) C[ch][a][b] += sum_e P[ch][e][a] Q[ch][e][b] in FP64, TR x TC
+      // register tiles (2 x 2 in the mixed form); the mixed form's FP32
+      // factors are widened, so each product is exact and the sums are FP64
+      // as in the FP64 form
       if (MIXED) {
         const int hp = Lp >> 1;
         const int tiles = 2 * hp * hp;
@@ -693,27 +695,22 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
           const int cch = jt / (hp * hp);
           const int r = jt - cch * hp * hp;
           const int a = 2 * (r / hp), b = 2 * (r - (r / hp) * hp);
-          const V *P = s_P + cch * nb * Lp + a;
-          const V *Q = s_Q + cch * nb * Lp + b;
+          const float *P = reinterpret_cast<const float *>(s_P) + cch * nb * Lp + a;
+          const float *Q = reinterpret_cast<const float *>(s_Q) + cch * nb * Lp + b;
           double *C0 = s_C + cch * Lp * Lp + a * Lp + b;
-          double *C1 = s_C + cch * Lp * Lp + (a + 1) * Lp + b;
-          KSum c00, c01, c10, c11;
-          __builtin_memcpy(&c00, C0, 8);
-          __builtin_memcpy(&c01, C0 + 1, 8);
-          __builtin_memcpy(&c10, C1, 8);
-          __builtin_memcpy(&c11, C1 + 1, 8);
+          double2 m0 = *reinterpret_cast<const double2 *>(C0);
+          double2 m1 = *reinterpret_cast<const double2 *>(C0 + Lp);
           for (int e = 0; e < nvalid; ++e) {
-            const V2 p = *reinterpret_cast<const V2 *>(P + e * Lp);
-            const V2 q = *reinterpret_cast<const V2 *>(Q + e * Lp);
-            ksum_add(c00, float(p.x) * float(q.x));
-            ksum_add(c01, float(p.x) * float(q.y));
-            ksum_add(c10, float(p.y) * float(q.x));
-            ksum_add(c11, float(p.y) * float(q.y));
+            const float2 pv = *reinterpret_cast<const float2 *>(P + e * Lp);
+            const float2 qv = *reinterpret_cast<const float2 *>(Q + e * Lp);
+            const double p0 = pv.x, p1 = pv.y, q0 = qv.x, q1 = qv.y;
+            m0.x = fma(p0, q0, m0.x);
+            m0.y = fma(p0, q1, m0.y);
+            m1.x = fma(p1, q0, m1.x);
+            m1.y = fma(p1, q1, m1.y);
           }
-          __builtin_memcpy(C0, &c00, 8);
-          __builtin_memcpy(C0 + 1, &c01, 8);
-          __builtin_memcpy(C1, &c10, 8);
-          __builtin_memcpy(C1 + 1, &c11, 8);
+          *reinterpret_cast<double2 *>(C0) = m0;
+          *reinterpret_cast<double2 *>(C0 + Lp) = m1;
         }
       } else {
         constexpr int TR = SPFRG_TILE_R, TC = SPFRG_TILE_C;
@@ -762,16 +759,8 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
           const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
           for (int i = b + lane; i < e; i += 64) {
             const int idx = F.ovl_idx[i];
-            if (MIXED) {
-              KSum a0, a1;
-              __builtin_memcpy(&a0, s_C + idx, 8);
-              __builtin_memcpy(&a1, s_C + Lp * Lp + idx, 8);
-              ps += double(a0.s) + double(a0.c);
-              pd += double(a1.s) + double(a1.c);
-            } else {
-              ps += s_C[idx];
-              pd += s_C[Lp * Lp + idx];
-            }
+            ps += s_C[idx];
+            pd += s_C[Lp * Lp + idx];
           }
           ps = wave_sum(ps);
           pd = wave_sum(pd);
