@@ -106,7 +106,7 @@ CORES = {
 # 2.00; WRITE_SIZE reads them exactly, scale 1.00).  The profile summaries
 # under profiles/ hold the runs and the commands.
 TRAFFIC_KIB = {
-    ("cubic-j1j2-r6-nw64", "f64"): {"fetch_kib": 740179.0, "write_kib": 47865.1,
+    ("cubic-j1j2-r6-nw64", "f64"): {"fetch_kib": 776662.0, "write_kib": 153309.0,
                                     "fetch_scale": 2.00, "write_scale": 1.00},
     ("kagome-dm-r5-nw48", "f64"): {"fetch_kib": 21931400.0, "write_kib": 134064.0,
                                    "fetch_scale": 2.00, "write_scale": 1.00},

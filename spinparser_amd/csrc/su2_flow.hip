@@ -108,9 +108,12 @@ constexpr int kMaxBatchMixed = SPFRG_MAX_BATCH_MIXED;
 // support gathers of one evaluation point when U == 1: 0 = per access buffer,
 // 1 = the next buffer's four loads in flight while the current one is used,
 // 2 = all sixteen loads issued before any use (the FP32 gathers of the mixed
-// form fit this in the occupancy-4 register budget; FP64 does not)
+// form fit this in the occupancy-4 register budget; FP64 does not).  FP64
+// uses 1: 3 % faster than 0 at small cutoffs although the occupancy-4 budget
+// then spills 12 bytes per lane; 2 needs 155 VGPRs (occupancy 3) and is 15 %
+// slower
 #ifndef SPFRG_GATHER
-#define SPFRG_GATHER 0
+#define SPFRG_GATHER 1
 #endif
 #ifndef SPFRG_GATHER_MIXED
 #define SPFRG_GATHER_MIXED 2
