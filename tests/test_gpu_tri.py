@@ -125,3 +125,53 @@ def test_tri_kagome_dm_parity():
         _log("  dSigma %.3g  dGamma %.3g  correlations %.3g" % (e2, e4, ec))
         ctx.close()
         assert e2 < TOL and e4 < TOL and ec < TOL
+
+
+def _gpu_flow(p):
+    from spinparser_amd.capi import TriFlowContext
+    ctx = TriFlowContext(p)
+    v2, v4 = p.initial_state(np.float64)
+    ctx.set_state(float(p.cutoffs[0]), v2, v4)
+    recs = []
+    for k in range(len(p.cutoffs) - 1):
+        ctx.compute_step()
+        recs.append(ctx.measure_correlation())
+        assert not ctx.is_diverged()
+        ctx.finalize_step(float(p.cutoffs[k + 1]))
+    recs.append(ctx.measure_correlation())
+    ctx.close()
+    return np.array(recs)
+
+
+def test_tri_dm_flow_spin_symmetries():
+    """Kagome J = 1 with out-of-plane DM D = +-0.3, range 3, 8 frequencies, whole flows on the GPU.
+
+    The kernel's own records must respect the spin symmetries of the model
+    (rotations about z: XX = YY, XY = -YX, XZ = YZ = ZX = ZY = 0; the pi rotation
+    about x maps D onto -D: XY, YX, XZ, ZX odd in D, the rest even), and agree with
+    the FP64 oracle's flow within the 1e-10 bar."""
+    from helpers import tri_problem_of
+    from oracle import TRI_GROUPS, TriOracle
+    flows = {}
+    for d in (0.3, -0.3):
+        p = tri_problem_of("kagome", 3, "kagome-DM", {"j": 1.0, "d": d}, 8, 0.05, 20.0, 10.0, 0.8, 0.9)
+        _log("TRI kagome-DM r3 D=%g L=%d O=%d, %d cutoffs" % (d, p.L, p.overlap_total, len(p.cutoffs)))
+        rec = _gpu_flow(p)
+        rec_o, n_o, _ = TriOracle(p).run_flow(dtype=np.float64, nthreads=8)
+        assert rec.shape[0] == n_o
+        worst = max(max_rel(rec[k], rec_o[k]) for k in range(n_o))
+        _log("  vs FP64 oracle %.3g" % worst)
+        assert worst < TOL
+        flows[d] = rec
+    g = {name: i for i, name in enumerate(TRI_GROUPS)}
+    a, b = flows[0.3], flows[-0.3]
+    tol = 1e-12 * np.abs(a).max()
+    assert np.abs(a[:, g["TRICorXY"]]).max() > 1e-3 * np.abs(a).max()
+    assert np.abs(a[:, g["TRICorXX"]] - a[:, g["TRICorYY"]]).max() < tol
+    assert np.abs(a[:, g["TRICorXY"]] + a[:, g["TRICorYX"]]).max() < tol
+    for name in ("TRICorXZ", "TRICorYZ", "TRICorZX", "TRICorZY"):
+        assert np.abs(a[:, g[name]]).max() < tol, name
+    parity = {"TRICorDD": 1, "TRICorXX": 1, "TRICorXY": -1, "TRICorXZ": -1, "TRICorYX": -1,
+              "TRICorYY": 1, "TRICorYZ": 1, "TRICorZX": -1, "TRICorZY": 1, "TRICorZZ": 1}
+    for name, sgn in parity.items():
+        assert np.abs(a[:, g[name]] - sgn * b[:, g[name]]).max() < tol, name

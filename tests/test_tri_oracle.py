@@ -7,7 +7,14 @@
 * TRI on a Heisenberg model equals the SU(2) core (test/scripted/test_reference1.sh:71-78):
   XX = YY = ZZ = SU2CorZZ and DD = SU2CorDD;
 * structure of the Pauli-derived coefficient tables: term counts, and the
-  density-density particle-particle row, whose form is fixed by sigma^a sigma^a = 1.
+  density-density particle-particle row, whose form is fixed by sigma^a sigma^a = 1;
+* antisymmetric (Dzyaloshinskii-Moriya) couplings, which no reference fixture
+  holds, through the spin symmetries they must respect: Heisenberg plus an
+  out-of-plane DM term is invariant under spin rotations about z, so every
+  correlation obeys XX = YY, XY = -YX and XZ = YZ = ZX = ZY = 0; and the global
+  pi rotation about x maps the model at D onto the model at -D, so XX, YY, ZZ,
+  DD, YZ, ZY are even in D and XY, YX, XZ, ZX odd.  A wrong sign, phase or
+  component index in the off-diagonal coupling algebra breaks these relations.
 """
 import numpy as np
 import pytest
@@ -90,3 +97,32 @@ def test_tri_tables_structure():
         assert set(np.unique(np.abs(t))) <= {0.0, 1.0}
     t, _ = TriOracle.table(4)
     assert set(np.unique(np.abs(t))) == {0.0, 2.0}
+
+
+def test_tri_dm_spin_symmetries():
+    """Kagome J = 1 with out-of-plane DM (model kagome-DM of spinparser_amd/res), range 3."""
+    from helpers import tri_problem_of
+    flows = {}
+    for d in (0.3, -0.3):
+        p = tri_problem_of("kagome", 3, "kagome-DM", {"j": 1.0, "d": d}, 8, 0.05, 20.0, 10.0, 0.8, 0.9)
+        rec, n, _ = TriOracle(p).run_flow(dtype=np.float64, nthreads=8)
+        assert n == len(p.cutoffs) - 1 or n == len(p.cutoffs)
+        flows[d] = rec[:n]
+    g = {name: i for i, name in enumerate(TRI_GROUPS)}
+    a, b = flows[0.3], flows[-0.3]
+    assert a.shape == b.shape
+    scale = np.abs(a).max()
+    tol = 1e-12 * scale
+    # the DM term is felt: the off-diagonal response is non-zero and SU(2) is broken
+    assert np.abs(a[:, g["TRICorXY"]]).max() > 1e-3 * scale
+    assert np.abs(a[:, g["TRICorXX"]] - a[:, g["TRICorZZ"]]).max() > 1e-4 * scale
+    # U(1) about z
+    assert np.abs(a[:, g["TRICorXX"]] - a[:, g["TRICorYY"]]).max() < tol
+    assert np.abs(a[:, g["TRICorXY"]] + a[:, g["TRICorYX"]]).max() < tol
+    for name in ("TRICorXZ", "TRICorYZ", "TRICorZX", "TRICorZY"):
+        assert np.abs(a[:, g[name]]).max() < tol, name
+    # pi rotation about x: D -> -D
+    parity = {"TRICorDD": 1, "TRICorXX": 1, "TRICorXY": -1, "TRICorXZ": -1, "TRICorYX": -1,
+              "TRICorYY": 1, "TRICorYZ": 1, "TRICorZX": -1, "TRICorZY": 1, "TRICorZZ": 1}
+    for name, sgn in parity.items():
+        assert np.abs(a[:, g[name]] - sgn * b[:, g[name]]).max() < tol, name
