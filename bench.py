@@ -115,6 +115,18 @@ TRAFFIC_KIB = {
 }
 
 
+def host_threads():
+    """CPU threads of the baselines: OMP_NUM_THREADS when set, else the CPUs
+    this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def build_problem(name):
     xml = make_task_xml(symmetry="SU2", **WORKLOADS[name])
     return su2_problem(parse_task(xml, is_text=True))
@@ -142,7 +154,7 @@ def cpu_baseline(prob, target_s=15.0):
     integrand-call counts) is extrapolated to the whole schedule."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = host_threads()
     o = Oracle(prob)
     v2, v4s, v4d = prob.initial_state(np.float32)
     dv2 = np.zeros(prob.nw, dtype=np.float32)
@@ -169,7 +181,8 @@ def cpu_baseline(prob, target_s=15.0):
     flow_flops = sum(evaluation_flops(prob, prob.cutoffs[k]) for k in range(ncut - 1))
     flow_s = sec_per_flop * flow_flops
     return {
-        "value": (ncut - 1) / flow_s, "unit": "dGamma/dLambda evals/s", "cores": threads, "kind": "port",
+        "value": (ncut - 1) / flow_s, "unit": "dGamma/dLambda evals/s", "cores": threads,
+        "cpu_model": cpu_model(), "kind": "port",
         "sample": "float32 CPU oracle (reference arithmetic, OpenMP guided over items) on %d items at each of "
                   "%d cutoffs (%.1f s measured, %.3g GFLOP/s); full-flow time %.0f s extrapolated by the "
                   "algorithmic FLOP of all %d evaluations" % (items_tot // len(ks), len(ks), t_tot,
@@ -342,7 +355,7 @@ def core_cpu_baseline(core, target_s=10.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     c = CORES[core]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = host_threads()
     p = build_core_problem(core)
     o = getattr(oracle, c["oracle"])(p)
     v2, v4 = p.initial_state(np.float32)
@@ -364,7 +377,8 @@ def core_cpu_baseline(core, target_s=10.0):
             n_items = int(min(p.n_freq // 2, n_items * max(1.0, target_s / max(t_tot, 1e-3))))
     flow_flops = sum(evaluation_flops(p, p.cutoffs[k]) for k in range(ncut - 1))
     flow_s = t_tot / f_tot * flow_flops
-    return {"value": (ncut - 1) / flow_s, "unit": "dGamma/dLambda evals/s", "cores": threads, "kind": "port",
+    return {"value": (ncut - 1) / flow_s, "unit": "dGamma/dLambda evals/s", "cores": threads,
+        "cpu_model": cpu_model(), "kind": "port",
             "sample": "float32 %s CPU oracle on %d items at each of 5 cutoffs (%.1f s measured, %.3g GFLOP/s); "
                       "full flow %.0f s extrapolated by algorithmic FLOP" % (c["label"], n_items, t_tot,
                                                                           f_tot / t_tot / 1e9, flow_s)}
@@ -410,7 +424,7 @@ def mode_core_flow(args, core):
            "dtype": "f64", "data": "synthetic (task-file generated flow, initial vertex = model couplings)",
            "config": {"workload": c["workload"], "L": p.L, "nw": p.nw, "overlap_terms": p.overlap_total,
                       "cutoffs": len(p.cutoffs), "step": "computeStep + correlation + isDiverged + finalizeStep"},
-           "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_entry(c["workload"], "f64"),
                         "kernel": c["kernel"],
                         "launches": dg_n, "avg_launch_ms": dg_ms / max(dg_n, 1),
@@ -421,17 +435,140 @@ def mode_core_flow(args, core):
     print(json.dumps(out))
 
 
+def sample_indices(nsteps_flow, steps):
+    """`steps` cutoff indices stratified over the schedule: the midpoints of
+    `steps` equal strata of the evaluation indices 0 .. nsteps_flow - 1."""
+    return [min(nsteps_flow - 1, int((i + 0.5) * nsteps_flow / steps)) for i in range(steps)]
+
+
+def stratified_leg(prob, precision, steps, warmup, label):
+    """Exactly `steps` iterations of the flow loop (computeStep, correlation,
+    isDiverged, finalizeStep; one ABI call per stage) at cutoffs stratified
+    over the whole schedule, each on the state the flow itself reaches there.
+
+    An untimed pass of the flow (sp_run_flow) stops at every sampled cutoff
+    and copies its state into a context of its own, so the timed region holds
+    the K evaluations only.  The time of an evaluation depends on the cutoff
+    (its integration windows), not on the vertex values; the full-schedule time
+    is the algorithmic FLOP of every evaluation times the measured seconds per
+    FLOP, interpolated linearly in the cutoff index between the samples."""
+    nsteps_flow = len(prob.cutoffs) - 1
+    ks = sample_indices(nsteps_flow, steps)
+    walker = FlowContext(prob, precision)
+    walker.set_state(float(prob.cutoffs[0]), *prob.initial_state(np.float64))
+    ctxs, pos = [], 0
+    for k in ks:
+        if k > pos:
+            _, div = walker.run_flow(prob.cutoffs[pos:k + 1])
+            if div:
+                raise RuntimeError("%s: flow diverged before cutoff index %d" % (label, k))
+            pos = k
+        c = FlowContext(prob, precision)
+        c.set_state(*walker.get_state())
+        ctxs.append(c)
+    for _ in range(warmup):  # untimed: the walker sits at the last sampled cutoff
+        walker.compute_step()
+        walker.measure_correlation()
+        walker.is_diverged()
+    walker.close()
+    step_flops = [evaluation_flops(prob, prob.cutoffs[k]) for k in range(nsteps_flow)]
+    for c in ctxs:
+        c.synchronize()
+        c.profile(True)
+    t_steps = []
+    t0 = time.perf_counter()
+    for c, k in zip(ctxs, ks):
+        ts = time.perf_counter()
+        c.compute_step()
+        c.measure_correlation()
+        if c.is_diverged():
+            raise RuntimeError("%s: flow diverged at cutoff index %d" % (label, k))
+        c.finalize_step(float(prob.cutoffs[k + 1]))
+        c.synchronize()
+        t_steps.append(time.perf_counter() - ts)
+    elapsed = time.perf_counter() - t0
+    dg_ms, dg_n = 0.0, 0
+    for c in ctxs:
+        ms, n = c.profile_get(KERNEL_DGAMMA)
+        dg_ms += ms
+        dg_n += n
+        c.close()
+    print("# %s: %d stratified steps %.3f s" % (label, steps, elapsed), file=sys.stderr, flush=True)
+    spf = np.array([t / step_flops[k] for t, k in zip(t_steps, ks)])
+    est = float(np.sum(np.array(step_flops) * np.interp(np.arange(nsteps_flow), ks, spf)))
+    return dict(elapsed=elapsed, dg_ms=dg_ms, dg_n=dg_n, flops=float(sum(step_flops[k] for k in ks)),
+                est_full_s=est, indices=ks)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def records_max_rel(ra, rb):
+    """Per-channel max over records of the normwise relative error."""
+    n = min(len(ra), len(rb))
+    return [max(max_rel(a[c], b[c]) for a, b in zip(ra[:n], rb[:n])) for c in range(2)]
+
+
+def square_leg(args):
+    """The CPU-runnable configuration (square lattice J = 1, range 4, 32
+    frequencies in [0.005, 50], cutoff 50 -> 0.01 x 0.98, 421 evaluations) end
+    to end: the FP64 GPU flow and the float CPU oracle flow (the reference's
+    arithmetic, OpenMP guided over items), each the whole cutoff loop with its
+    correlation records."""
+    prob = build_problem("square-r4-nw32")
+    nsteps = len(prob.cutoffs) - 1
+    r = flow_leg(prob, SP_PRECISION_FP64, nsteps, 2, "square fp64")
+    out = {"workload": "square-r4-nw32", "L": prob.L, "nw": prob.nw, "n_freq_triples": prob.n_freq,
+           "cutoffs": len(prob.cutoffs), "gpu_evals_per_s": nsteps / r["elapsed"], "gpu_flow_s": r["elapsed"],
+           "gpu_k_dgamma_avg_ms": r["dg_ms"] / max(r["dg_n"], 1),
+           "gpu_tflops_k_dgamma": r["flops"] / (r["dg_ms"] * 1e-3) / 1e12}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        threads = host_threads()
+        t0 = time.perf_counter()
+        zz, dd, n, _ = Oracle(prob).run_flow(np.float32, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        cpu_recs = [(zz[k].astype(np.float64), dd[k].astype(np.float64)) for k in range(min(n, nsteps))]
+        ezz, edd = records_max_rel(r["records"], cpu_recs)
+        out["cpu_baseline"] = {"value": nsteps / cpu_s, "unit": "dGamma/dLambda evals/s", "cores": threads,
+                               "cpu_model": cpu_model(), "kind": "port", "flow_s": cpu_s,
+                               "sample": "the whole float32 CPU oracle flow (reference arithmetic, OpenMP guided "
+                                         "over items), %d evaluations and correlation records" % nsteps}
+        out["gpu_vs_cpu"] = out["gpu_evals_per_s"] / out["cpu_baseline"]["value"]
+        out["max_rel_err_corr_gpu_fp64_vs_cpu_float"] = {"zz": ezz, "dd": edd}
+    return out
+
+
 def mode_flow(args):
     prob = build_problem(args.workload)
     nsteps_flow = len(prob.cutoffs) - 1
-    r64 = flow_leg(prob, SP_PRECISION_FP64, args.steps, args.warmup, "fp64")
-    achieved = r64["flops"] / (r64["dg_ms"] * 1e-3) / 1e12
+    stratified = args.steps < nsteps_flow
+    if stratified:
+        r64 = stratified_leg(prob, SP_PRECISION_FP64, args.steps, args.warmup, "fp64")
+        value = nsteps_flow / r64["est_full_s"]
+        how = ("%d evaluations stratified over the %d-evaluation schedule (each on the flow's own state at that "
+               "cutoff), full-schedule time = algorithmic FLOP of every evaluation x measured s/FLOP interpolated "
+               "over the cutoff index" % (args.steps, nsteps_flow))
+    else:
+        r64 = flow_leg(prob, SP_PRECISION_FP64, args.steps, args.warmup, "fp64")
+        value = args.steps / r64["elapsed"]
+        how = "whole flows from the first cutoff, %d evaluations timed" % args.steps
     elapsed = r64["elapsed"]
-    hbm_gbs = compulsory_bytes(prob) * args.steps / elapsed / 1e9
+    achieved = r64["flops"] / (r64["dg_ms"] * 1e-3) / 1e12
     traffic = traffic_entry(args.workload, "f64")
+    avg_ms = r64["dg_ms"] / max(r64["dg_n"], 1)
     out = {
         "metric": "dGamma/dLambda evals/s (full-schedule average, FP64 flow)",
-        "value": args.steps / elapsed,
+        "value": value,
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -445,40 +582,58 @@ def mode_flow(args):
         "config": {"workload": args.workload, "L": prob.L, "nw": prob.nw, "n_freq_triples": prob.n_freq,
                    "overlap_terms": prob.overlap_total, "cutoffs": len(prob.cutoffs),
                    "step": "computeStep + correlation + isDiverged + finalizeStep",
-                   "loop": "sp_run_flow (steps queued, records read back every step, divergence checked on the device)"},
-        "hbm_gbs_compulsory": hbm_gbs,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "timing": how},
+        "hbm_gbs_compulsory": compulsory_bytes(prob) / (avg_ms * 1e-3) / 1e9,
+        "roofline": {"bound": "valu_fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_dgamma", "launches": r64["dg_n"], "avg_launch_ms": r64["dg_ms"] / max(r64["dg_n"], 1),
+                     "kernel": "k_dgamma", "launches": r64["dg_n"], "avg_launch_ms": avg_ms,
                      "algorithmic_tflop_total": r64["flops"] / 1e12,
-                     "peak_note": "FP64 vector FMA peak (no MFMA: gathers and sparse site sums)"},
+                     "peak_note": "MI355X FP64 vector FMA peak (equal to the FP64 matrix peak); the kernel issues "
+                                  "no MFMA: gathers and sparse site sums",
+                     "window": "the timed evaluations above; traffic = full-flow mean HBM bytes per launch "
+                               "(rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"},
         "cpu_baseline": None,
     }
     if traffic is not None:
-        out["roofline"]["traffic_gbs"] = traffic / (out["roofline"]["avg_launch_ms"] * 1e-3) / 1e9
-    if not args.no_step_api:
+        out["roofline"]["traffic_gbs"] = traffic / (avg_ms * 1e-3) / 1e9
+    # the whole FP64 flow through sp_run_flow, beside the headline
+    full = r64 if not stratified else flow_leg(prob, SP_PRECISION_FP64, nsteps_flow, 0, "fp64 full flow")
+    out["full_flow"] = {"evals_per_s": nsteps_flow / full["elapsed"] if stratified else value,
+                        "flow_s": full["elapsed"], "k_dgamma_avg_ms": full["dg_ms"] / max(full["dg_n"], 1),
+                        "tflops_k_dgamma": full["flops"] / (full["dg_ms"] * 1e-3) / 1e12,
+                        "loop": "sp_run_flow (steps queued, records read back every step, divergence checked "
+                                "on the device)"}
+    out["full_flow"]["frac"] = out["full_flow"]["tflops_k_dgamma"] / FP64_PEAK_TFLOPS
+    if not args.no_step_api and not stratified:
         # the same flow with one ABI call per stage and step (the FrgCore call pattern)
         rs = flow_leg(prob, SP_PRECISION_FP64, args.steps, 0, "fp64 step api", queued=False)
-        same = (len(rs["records"]) == len(r64["records"]) and
-                all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(rs["records"], r64["records"])))
         out["step_api"] = {"evals_per_s": args.steps / rs["elapsed"], "ms_per_step": rs["elapsed"] * 1e3 / args.steps,
-                           "records_identical_to_queued": bool(same)}
+                           "records_identical_to_queued": bool(
+                               len(rs["records"]) == len(r64["records"]) and
+                               all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+                                   for a, b in zip(rs["records"], r64["records"])))}
     if not args.no_fixed:
         out["fixed_cutoff"] = fixed_cutoff(prob, SP_PRECISION_FP64)
     if not args.no_mixed:
-        r32 = flow_leg(prob, SP_PRECISION_MIXED, args.steps, 0, "mixed")
-        mixed = {"evals_per_s": args.steps / r32["elapsed"], "ms_per_step": r32["elapsed"] * 1e3 / args.steps,
-                 "k_dgamma_avg_ms": r32["dg_ms"] / max(r32["dg_n"], 1),
-                 "speedup_vs_fp64": elapsed / r32["elapsed"]}
-        if r64["records"] and len(r64["records"]) == len(r32["records"]):
-            mixed["max_rel_err_corr_zz"] = max(max_rel(a[0], b[0]) for a, b in zip(r32["records"], r64["records"]))
-            mixed["max_rel_err_corr_dd"] = max(max_rel(a[1], b[1]) for a, b in zip(r32["records"], r64["records"]))
-            mixed["records_compared"] = len(r64["records"])
-        if r64["final"] is not None and r32["final"] is not None:
+        if stratified:
+            m = stratified_leg(prob, SP_PRECISION_MIXED, args.steps, 0, "mixed")
+            mrate = nsteps_flow / m["est_full_s"]
+        else:
+            m = flow_leg(prob, SP_PRECISION_MIXED, args.steps, 0, "mixed")
+            mrate = args.steps / m["elapsed"]
+        r32 = m if not stratified else flow_leg(prob, SP_PRECISION_MIXED, nsteps_flow, 0, "mixed full flow")
+        mixed = {"evals_per_s": mrate, "k_dgamma_avg_ms": m["dg_ms"] / max(m["dg_n"], 1),
+                 "speedup_vs_fp64": mrate / value,
+                 "full_flow_evals_per_s": nsteps_flow / r32["elapsed"] if stratified else mrate}
+        if full["records"] and len(full["records"]) == len(r32["records"]):
+            mixed["max_rel_err_corr_zz"], mixed["max_rel_err_corr_dd"] = records_max_rel(r32["records"],
+                                                                                       full["records"])
+            mixed["records_compared"] = len(full["records"])
+        if full["final"] is not None and r32["final"] is not None:
             names = ("Sigma", "Gamma_spin", "Gamma_density")
             mixed["max_rel_err_final_state"] = {n: max_rel(a, b) for n, a, b in
-                                                zip(names, r32["final"][1:], r64["final"][1:])}
-            mixed["max_rel_err_derivative_final_state"] = derivative_errors(prob, r64["final"])
+                                                zip(names, r32["final"][1:], full["final"][1:])}
+            mixed["max_rel_err_derivative_final_state"] = derivative_errors(prob, full["final"])
         init = prob.initial_state(np.float64)
         mid = float(prob.cutoffs[(nsteps_flow - 1) // 2])
         mixed["max_rel_err_derivative_initial_state_mid_cutoff"] = derivative_errors(prob, (mid, *init))
@@ -486,81 +641,109 @@ def mode_flow(args):
     legs = [c for c, off in (("tri", args.no_tri), ("xyz", args.no_xyz)) if not off]
     for c in legs:
         out[c] = core_leg(c)
+    if not args.no_square:
+        out["square"] = square_leg(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
         for c in legs:
             out[c]["cpu_baseline"] = core_cpu_baseline(c, min(10.0, args.cpu_seconds))
+    if not args.no_sweep:
+        out["sweep"] = sweep_leg(args)
     print(json.dumps(out))
 
 
-def mode_sweep(args):
-    from spinparser_amd.serve import SweepServer, run_alone, sweep_requests
+def sweep_leg(args):
+    """The served J2/J1 sweep against the same flows run one at a time in FP64
+    (sp_run_flow).  Returns throughput, latency, and the correlation error of
+    every served flow against its FP64 run."""
+    from spinparser_amd.serve import PrecisionSchedule, SweepServer, run_alone, sweep_requests
     t0 = time.perf_counter()
     reqs = sweep_requests(args.flows, seed=args.seed, rate_per_s=args.rate)
-    t_build = time.perf_counter() - t0
-    print("# %d requests built in %.1f s" % (len(reqs), t_build), file=sys.stderr, flush=True)
-    fps = [footprint_bytes(r.problem, SP_PRECISION_MIXED) for r in reqs]
+    print("# %d sweep requests built in %.1f s" % (len(reqs), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    precision = SP_PRECISION_FP64 if args.sweep_fp64 else SP_PRECISION_MIXED
+    schedule = None if (args.sweep_fp64 or args.fp64_below <= 0) else PrecisionSchedule(args.fp64_below)
+    fps = [footprint_bytes(r.problem, precision) for r in reqs]
     budget = min(HBM_BYTES - (16 << 30), int(args.budget_gb * (1 << 30)))
-    srv = SweepServer(budget_bytes=budget, max_flows=args.flows, precision=SP_PRECISION_MIXED)
+    srv = SweepServer(budget_bytes=budget, max_flows=args.flows, precision=precision, schedule=schedule)
     t0 = time.perf_counter()
     served = srv.run(reqs, log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     t_served = time.perf_counter() - t0
     srv.close()
     lat = np.array([r.latency for r in served])
+    form = ("FP64" if precision == SP_PRECISION_FP64 else
+            "mixed" if schedule is None else
+            "mixed above %.3g J, FP64 below" % schedule.fp64_below)
+    out = {"workload": "cubic-j1j2-sweep-%d" % args.flows, "seed": args.seed, "rate_per_s": args.rate,
+           "precision": form, "flows": len(reqs),
+           "flows_per_hour": len(served) / t_served * 3600.0, "sweep_wall_s": t_served,
+           "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+           "evaluations": int(sum(r.steps for r in served)),
+           "fp64_fraction_of_evaluations": srv.fp64_steps / max(1, srv.fp64_steps + srv.mixed_steps),
+           "budget_bytes": budget, "footprint_total_bytes": int(sum(fps)), "footprint_max_bytes": int(max(fps)),
+           "peak_flows_resident": srv.peak_flows, "peak_bytes_resident": srv.peak_bytes,
+           "mean_batch": float(np.mean(srv.batch_sizes)), "diverged": int(sum(r.diverged for r in served))}
+    if args.no_alone:
+        return out
+    t_alone = 0.0
+    worst = {"zz": 0.0, "dd": 0.0, "flow": None}
+    worst_div = {"zz": 0.0, "dd": 0.0}
+    same_end = 0
+    per_flow = []
+    for r, sv in zip(reqs, served):
+        a = run_alone(r, SP_PRECISION_FP64, queued=True)
+        t_alone += a.latency
+        same = len(a.zz) == len(sv.zz) and a.diverged == sv.diverged
+        same_end += int(same)
+        n = min(len(a.zz), len(sv.zz))
+        # a flow that diverges has its final records dominated by the blow-up,
+        # where relative comparisons carry no information: its records up to
+        # ten before the divergence are reported separately
+        n_cmp = max(0, n - 10) if (a.diverged or sv.diverged) else n
+        ezz = max((max_rel(x, y) for x, y in zip(sv.zz[:n_cmp], a.zz[:n_cmp])), default=0.0)
+        edd = max((max_rel(x, y) for x, y in zip(sv.dd[:n_cmp], a.dd[:n_cmp])), default=0.0)
+        if a.diverged or sv.diverged:
+            worst_div["zz"], worst_div["dd"] = max(worst_div["zz"], ezz), max(worst_div["dd"], edd)
+        elif max(ezz, edd) > max(worst["zz"], worst["dd"]):
+            worst = {"zz": max(worst["zz"], ezz), "dd": max(worst["dd"], edd), "flow": r.ident}
+        else:
+            worst["zz"], worst["dd"] = max(worst["zz"], ezz), max(worst["dd"], edd)
+        per_flow.append({"flow": r.ident, "j2": round(r.params["j2"], 4), "range": r.params["range"],
+                         "nw": r.params["nw"], "records": len(sv.zz), "records_fp64": len(a.zz),
+                         "diverged": bool(sv.diverged), "max_rel_err": max(ezz, edd)})
+        print("# FP64 alone flow %d %s: %.2f s, %d records, diverged %s (served: %d, %s), err %.2e" % (
+            r.ident, r.params, a.latency, len(a.zz), a.diverged, len(sv.zz), sv.diverged, max(ezz, edd)),
+            file=sys.stderr, flush=True)
+    out["fp64_one_at_a_time"] = {"wall_s": t_alone, "flows_per_hour": len(reqs) / t_alone * 3600.0,
+                                 "loop": "sp_run_flow, one flow after another"}
+    out["speedup_vs_fp64_one_at_a_time"] = out["flows_per_hour"] / out["fp64_one_at_a_time"]["flows_per_hour"]
+    if precision == SP_PRECISION_MIXED:
+        out["max_rel_err_corr_vs_fp64"] = {"zz": worst["zz"], "dd": worst["dd"], "worst_flow": worst["flow"],
+                                           "flows": len(reqs) - int(sum(r.diverged for r in served))}
+        out["max_rel_err_corr_vs_fp64_diverging_flows_before_last_10"] = worst_div
+    out["flows_same_end_as_fp64"] = same_end
+    out["per_flow"] = per_flow
+    return out
+
+
+def mode_sweep(args):
+    sw = sweep_leg(args)
     out = {
-        "metric": "served sweep flows/hour (mixed precision, continuous batching)",
-        "value": len(served) / t_served * 3600.0,
+        "metric": "served sweep flows/hour (continuous batching, %s)" % sw["precision"],
+        "value": sw["flows_per_hour"],
         "unit": "flows/hour",
         "n_gpus": 1,
-        "steps": int(sum(r.steps for r in served)),
+        "steps": sw["evaluations"],
         "warmup": 0,
-        "ms_per_step": t_served * 1e3 / max(1, sum(r.steps for r in served)),
+        "ms_per_step": sw["sweep_wall_s"] * 1e3 / max(1, sw["evaluations"]),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64" if args.sweep_fp64 else "f32+f64",
         "data": "synthetic (cubic J1-J2 flows drawn from PCG64 seed %d)" % args.seed,
-        "config": {"workload": "cubic-j1j2-sweep-%d" % args.flows, "seed": args.seed, "rate_per_s": args.rate,
-                   "ranges": [3, 4, 5, 6, 7], "nws": [32, 48, 64], "budget_bytes": budget,
-                   "footprint_total_bytes": int(sum(fps)), "footprint_max_bytes": int(max(fps))},
-        "latency_p50_s": float(np.percentile(lat, 50)),
-        "latency_p99_s": float(np.percentile(lat, 99)),
-        "sweep_wall_s": t_served,
-        "peak_flows_resident": srv.peak_flows,
-        "peak_bytes_resident": srv.peak_bytes,
-        "mean_batch": float(np.mean(srv.batch_sizes)),
-        "diverged": int(sum(r.diverged for r in served)),
+        "config": {"workload": sw["workload"], "seed": args.seed, "rate_per_s": args.rate,
+                   "ranges": [3, 4, 5, 6, 7], "nws": [32, 48, 64]},
+        "sweep": sw,
     }
-    if not args.no_alone:
-        t_alone = 0.0
-        worst_zz = worst_dd = 0.0
-        worst_zz_div = worst_dd_div = 0.0
-        same_end = 0
-        for r, s in zip(reqs, served):
-            a = run_alone(r, SP_PRECISION_FP64)
-            t_alone += a.latency
-            same_end += int(len(a.zz) == len(s.zz) and a.diverged == s.diverged)
-            # records up to the last one before either run diverged: a flow
-            # that diverges has its final records dominated by the blow-up,
-            # where relative comparisons carry no information
-            n = min(len(a.zz), len(s.zz))
-            if a.diverged or s.diverged:
-                n_ok = max(0, n - 10)
-                if n_ok:
-                    worst_zz_div = max(worst_zz_div, max(max_rel(x, y) for x, y in zip(s.zz[:n_ok], a.zz[:n_ok])))
-                    worst_dd_div = max(worst_dd_div, max(max_rel(x, y) for x, y in zip(s.dd[:n_ok], a.dd[:n_ok])))
-            else:
-                worst_zz = max(worst_zz, max(max_rel(x, y) for x, y in zip(s.zz[:n], a.zz[:n])))
-                worst_dd = max(worst_dd, max(max_rel(x, y) for x, y in zip(s.dd[:n], a.dd[:n])))
-            print("# alone FP64 flow %d %s: %.2f s, %d records, diverged %s (served: %d, %s)" % (
-                r.ident, r.params, a.latency, len(a.zz), a.diverged, len(s.zz), s.diverged),
-                file=sys.stderr, flush=True)
-        out["fp64_one_at_a_time"] = {"wall_s": t_alone, "flows_per_hour": len(reqs) / t_alone * 3600.0}
-        out["speedup_vs_fp64_one_at_a_time"] = out["value"] / out["fp64_one_at_a_time"]["flows_per_hour"]
-        out["max_rel_err_corr_vs_fp64"] = {"zz": worst_zz, "dd": worst_dd,
-                                           "flows": len(reqs) - int(sum(r.diverged for r in served))}
-        out["max_rel_err_corr_vs_fp64_diverging_flows_before_last_10"] = {"zz": worst_zz_div, "dd": worst_dd_div}
-        out["flows_same_end_as_fp64"] = same_end
     print(json.dumps(out))
 
 
@@ -583,6 +766,11 @@ def main():
     ap.add_argument("--rate", type=float, default=1.0, help="Poisson arrival rate of the sweep (flows/s)")
     ap.add_argument("--budget-gb", type=float, default=256.0)
     ap.add_argument("--no-alone", action="store_true")
+    ap.add_argument("--no-square", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-fp64", action="store_true", help="serve the sweep in the FP64 form")
+    ap.add_argument("--fp64-below", type=float, default=3.0,
+                    help="mixed sweep: FP64 form at cutoffs below this multiple of the coupling scale (0: never)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU and is measured on one GPU only; "

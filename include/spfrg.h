@@ -76,6 +76,17 @@ int sp_ctx_destroy(sp_ctx *ctx);
  * readCheckpoint do (src/SU2/SU2EffectiveAction.hpp:38-60,159-205). */
 int sp_set_state(sp_ctx *ctx, double cutoff, const double *v2, const double *v4ss, const double *v4dd);
 
+/* Switch the precision form of a context between SP_PRECISION_FP64 and
+ * SP_PRECISION_MIXED; no reference counterpart (the reference computes in
+ * float throughout).  The FP64 state is the master copy in both forms, so a
+ * switch changes how later derivatives are evaluated, never the state.  The
+ * mixed form needs the FP32 shadow, which a context created with
+ * SP_PRECISION_MIXED holds; switching to it rebuilds the shadow from the
+ * state.  A served flow switches to the FP64 form over the cutoff window in
+ * which its correlations are most sensitive to the FP32 rounding.
+ * Asynchronous on the context stream. */
+int sp_set_precision(sp_ctx *ctx, int precision);
+
 /* Download the state; the read side of SU2EffectiveAction::writeCheckpoint
  * (src/SU2/SU2EffectiveAction.hpp:79-150).  Null pointers skip that array. */
 int sp_get_state(sp_ctx *ctx, double *cutoff, double *v2, double *v4ss, double *v4dd);

@@ -54,6 +54,10 @@ int sp_ckpt_write(const char *path, int append, float cutoff, int narrays, const
                   const long long *sizes, const float *const *data);
 int sp_ckpt_read(const char *path, int id, float *cutoff, int narrays, const char *const *names,
                  const long long *sizes, float *const *data);
+/* Number of checkpoints in a data file (0 if the file is absent); the
+ * post-processing stage reads checkpoints 0, 1, ... until readCheckpoint
+ * reports false (src/SpinParser.cpp:192-197). */
+int sp_ckpt_count(const char *path);
 
 /* Message of the last failure in this thread. */
 const char *sp_obs_last_error(void);

@@ -29,6 +29,18 @@ class OrProblem(ctypes.Structure):
                 ("spin", ctypes.c_double)]
 
 
+def host_threads() -> int:
+    """OpenMP threads for the oracle: OMP_NUM_THREADS when set, else the CPUs
+    this process may run on (not every CPU of the machine)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -66,7 +78,7 @@ class Oracle:
 
     def compute_step(self, cutoff, v2, v4s, v4d, dtype=np.float64, nthreads=None):
         """One right-hand-side evaluation; returns (dv2, dv4s, dv4d, calls[S,T,U])."""
-        nthreads = nthreads or os.cpu_count() or 1
+        nthreads = nthreads or host_threads()
         p = self.prob
         v2, v4s, v4d = (np.ascontiguousarray(x, dtype=dtype) for x in (v2, v4s, v4d))
         dv2 = np.zeros(p.nw, dtype=dtype)
@@ -87,7 +99,7 @@ class Oracle:
 
     def items(self, cutoff, v2, dv2, v4s, v4d, it0, it1, dtype=np.float64, nthreads=None):
         """dGamma for items [it0, it1) only (bounded CPU timing samples)."""
-        nthreads = nthreads or os.cpu_count() or 1
+        nthreads = nthreads or host_threads()
         p = self.prob
         dss = np.zeros(p.vertex_size, dtype=dtype)
         ddd = np.zeros(p.vertex_size, dtype=dtype)
@@ -123,7 +135,7 @@ class Oracle:
 
     def run_flow(self, dtype=np.float32, nthreads=None, ncut=None):
         """Full flow over the cutoff schedule; returns (zz[n,B,Rb], dd[n,B,Rb], n_records, state)."""
-        nthreads = nthreads or os.cpu_count() or 1
+        nthreads = nthreads or host_threads()
         p = self.prob
         cut = np.ascontiguousarray(p.cutoffs if ncut is None else p.cutoffs[:ncut], dtype=np.float32)
         v2, v4s, v4d = p.initial_state(dtype)
@@ -203,7 +215,7 @@ class TriOracle:
 
     def compute_step(self, cutoff, v2, v4, dtype=np.float64, nthreads=None):
         """(dv2, dv4, calls[S,T,U]) at one cutoff."""
-        nthreads = nthreads or os.cpu_count() or 1
+        nthreads = nthreads or host_threads()
         p = self.prob
         t, ct, sfx = self._t(dtype)
         v2, v4 = (np.ascontiguousarray(x, dtype=dtype) for x in (v2, v4))
@@ -216,7 +228,7 @@ class TriOracle:
         return dv2, dv4, calls
 
     def items(self, cutoff, v2, dv2, v4, it0, it1, dtype=np.float64, nthreads=None):
-        nthreads = nthreads or os.cpu_count() or 1
+        nthreads = nthreads or host_threads()
         t, ct, sfx = self._t(dtype)
         v2, dv2, v4 = (np.ascontiguousarray(x, dtype=dtype) for x in (v2, dv2, v4))
         dv4 = np.zeros(self.prob.vertex_size, dtype=dtype)
@@ -225,6 +237,14 @@ class TriOracle:
                                                   _ptr(v4, t), ctypes.c_int(it0), ctypes.c_int(it1), _ptr(dv4, t),
                                                   ctypes.c_int(nthreads), _ptr(calls, _lp))
         return dv4, calls
+
+    def dsigma(self, cutoff, v2, v4):
+        """dSigma/dLambda only (FP64; TRI core)."""
+        v2, v4 = (np.ascontiguousarray(x, dtype=np.float64) for x in (v2, v4))
+        dv2 = np.zeros(self.prob.nw)
+        getattr(self.lib, self._prefix + "dsigma_f64")(ctypes.byref(self.c), ctypes.c_double(float(cutoff)),
+                                                       _ptr(v2, _f64p), _ptr(v4, _f64p), _ptr(dv2, _f64p))
+        return dv2
 
     def correlation(self, cutoff, v2, v4, dtype=np.float64):
         """dict group -> (B, Rb) array of the correlation outputs (10 for TRI)."""
@@ -240,7 +260,7 @@ class TriOracle:
 
     def run_flow(self, dtype=np.float32, nthreads=None, ncut=None):
         """Full flow; returns (records[n, groups, B, Rb], n_records, (v2, v4))."""
-        nthreads = nthreads or os.cpu_count() or 1
+        nthreads = nthreads or host_threads()
         p = self.prob
         cut = np.ascontiguousarray(p.cutoffs if ncut is None else p.cutoffs[:ncut], dtype=np.float32)
         v2, v4 = p.initial_state(dtype)

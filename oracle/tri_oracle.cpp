@@ -634,6 +634,12 @@ void or_tri_compute_step_f32(const or_tri_problem *p, float cutoff, const float 
   tri_compute_step<float>(*p, cutoff, v2, v4, dv2, dv4, nthreads, calls);
 }
 
+// dSigma/dLambda only (FP64), for checks on large problems
+void or_tri_dsigma_f64(const or_tri_problem *p, double cutoff, const double *v2, const double *v4, double *dv2) {
+  TriCtx<double> c(*p, v2, nullptr, v4);
+  for (int i = 0; i < p->nw; ++i) dv2[i] = tri_vertex_single_particle(c, i, cutoff);
+}
+
 void or_tri_items_f64(const or_tri_problem *p, double cutoff, const double *v2, const double *dv2, const double *v4,
                       int it0, int it1, double *dv4, int nthreads, long *calls) {
   tri_items<double>(*p, cutoff, v2, dv2, v4, it0, it1, dv4, nthreads, calls);

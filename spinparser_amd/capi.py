@@ -27,7 +27,7 @@ EXPORTED_SYMBOLS = (
     "sp_ctx_create", "sp_ctx_destroy", "sp_set_state", "sp_get_state", "sp_get_flow", "sp_compute_step",
     "sp_finalize_step", "sp_is_diverged", "sp_measure_correlation", "sp_run_flow", "sp_synchronize", "sp_last_error",
     "sp_profile_enable", "sp_profile_get", "sp_ctx_footprint", "sp_group_create", "sp_group_destroy",
-    "sp_group_step", "sp_group_measure", "sp_group_last_error",
+    "sp_group_step", "sp_group_measure", "sp_group_last_error", "sp_set_precision",
 )
 
 
@@ -64,6 +64,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.sp_ctx_destroy.argtypes = [c_ctx]
     lib.sp_set_state.argtypes = [c_ctx, ctypes.c_double, _f64p, _f64p, _f64p]
     lib.sp_get_state.argtypes = [c_ctx, _f64p, _f64p, _f64p, _f64p]
+    lib.sp_set_precision.argtypes = [c_ctx, ctypes.c_int]
     lib.sp_get_flow.argtypes = [c_ctx, _f64p, _f64p, _f64p]
     lib.sp_compute_step.argtypes = [c_ctx]
     lib.sp_finalize_step.argtypes = [c_ctx, ctypes.c_double]
@@ -126,6 +127,7 @@ class FlowContext:
     def __init__(self, prob, precision: int = SP_PRECISION_FP64, lib: Optional[ctypes.CDLL] = None):
         self.lib = lib or load_library()
         self.prob = prob
+        self.precision = int(precision)
         self._buf = ProblemBuffers(prob, precision)
         self.ctx = ctypes.c_void_p()
         rc = self.lib.sp_ctx_create(ctypes.byref(self._buf.struct), ctypes.byref(self.ctx))
@@ -158,6 +160,12 @@ class FlowContext:
             raise FlowError(SP_EARG, "state arrays have the wrong size")
         self._check(self.lib.sp_set_state(self.ctx, float(cutoff), _p(v2, _f64p), _p(v4s, _f64p), _p(v4d, _f64p)),
                     "sp_set_state")
+
+    def set_precision(self, precision: int):
+        """Evaluate later derivatives in the FP64 or the mixed form (the state
+        stays FP64); the mixed form needs a context created mixed."""
+        self._check(self.lib.sp_set_precision(self.ctx, int(precision)), "sp_set_precision")
+        self.precision = int(precision)
 
     def get_state(self) -> Tuple[float, np.ndarray, np.ndarray, np.ndarray]:
         c = ctypes.c_double(0)

@@ -178,6 +178,15 @@ int sp_ckpt_write(const char *path, int append, float cutoff, int narrays, const
   return id;
 }
 
+int sp_ckpt_count(const char *path) {
+  if (!path) return err(SP_OBS_EARG, "invalid argument");
+  H5Eset_auto(H5E_DEFAULT, nullptr, nullptr);
+  if (H5Fis_hdf5(path) <= 0) return 0;
+  H file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+  if (file.id < 0) return err(SP_OBS_EIO, std::string("could not open data file ") + path);
+  return (int)measurement_names(file.id).size();
+}
+
 int sp_ckpt_read(const char *path, int id, float *cutoff, int narrays, const char *const *names,
                  const long long *sizes, float *const *data) {
   if (!path || !cutoff || narrays < 0 || (narrays > 0 && (!names || !sizes || !data)))

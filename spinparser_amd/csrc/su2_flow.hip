@@ -178,7 +178,9 @@ struct FlowDev {
   double *Cwp, *Cw;
   int *Cn;
   double *corr_partial, *corr_out;
-  int *flags;  // [0] NaN in the flow, [1] evaluation-list overflow
+  int *flags;  // [0] NaN in the flow, [1] evaluation-list overflow, [2] stopped flow,
+               // [4] scale exponent S of the FP32 shadow (G32 = G * 2^-S), [5] largest
+               // binary exponent of G written since the last plan, [6] planned S
   int nw, L, R, B, Rb, emax, nb, slots, precision, euler;
   long long nf, item_base, nv;
   unsigned g_bytes, g32_bytes;
@@ -797,16 +799,49 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
     for (int k = 0; k < slots; ++k) { ts += s_red[(k * L + tid) * 2]; td += s_red[(k * L + tid) * 2 + 1]; }
     ts *= (1.0 / kTwoPi);
     td *= (1.0 / kTwoPi);
+    if (MIXED) {
+      // undo the shadow's power-of-two scale (quadratic in the vertex)
+      const int S2 = 2 * F.flags[4];
+      ts = ldexp(ts, S2);
+      td = ldexp(td, S2);
+    }
     F.dG[(size_t)item * L + tid] = make_double2(ts, td);
     if (isnan(ts) || isnan(td)) atomicOr(&F.flags[0], 1);
   }
+}
+
+// The mixed form's FP32 shadow holds G * 2^-S.  Every term of dGamma is a
+// product of two vertex values with Gamma-independent coefficients
+// (src/SU2/SU2FrgCore.cpp:199-431), so the mixed dGamma kernel multiplies its
+// result by 2^(2S); the power-of-two scale is exact.  S is planned from the
+// largest binary exponent of G so that shadow values stay below 2^kShadowExp:
+// a flow whose vertex grows towards a divergence never overflows FP32.  S = 0
+// (no scaling) while |G| < 2^kShadowExp.
+constexpr int kShadowExp = 16;
+__device__ __forceinline__ int shadow_plan(int max_exp) {
+  return max_exp > kShadowExp ? (max_exp - kShadowExp < 1000 ? max_exp - kShadowExp : 1000) : 0;
+}
+// unbiased binary exponent of the larger of |a|, |b| (1024 for inf / NaN)
+__device__ __forceinline__ int pair_exp(double a, double b) {
+  const int ea = (int)((__double_as_longlong(a) >> 52) & 0x7ff);
+  const int eb = (int)((__double_as_longlong(b) >> 52) & 0x7ff);
+  return (ea > eb ? ea : eb) - 1023;
+}
+__device__ __forceinline__ void atomic_max_wave(int *dst, int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(dst, v);
 }
 
 // in-place explicit Euler step (src/SU2/SU2FrgCore.cpp:111-137); a flow whose
 // derivative holds a NaN is left untouched, as the reference stops before
 // finalizeStep (src/SpinParser.cpp:151-159), and flags[2] records that it
 // stopped: later steps enqueued behind it (sp_run_flow) leave it untouched
-// too.  Rewrites the FP32 shadow.
+// too.  A flow in the mixed form also rewrites the FP32 shadow with the
+// planned scale and records the largest exponent of the new G.
 __global__ void k_euler(const FlowDev *__restrict__ fl) {
   const FlowDev &F = fl[blockIdx.y];
   if (!F.euler) return;
@@ -819,27 +854,62 @@ __global__ void k_euler(const FlowDev *__restrict__ fl) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   double2 *G = F.G;
   const double2 *dG = F.dG;
-  float2 *G32 = F.G32;
+  float2 *G32 = F.precision == SP_PRECISION_MIXED ? F.G32 : nullptr;
+  const int S = G32 ? F.flags[6] : 0;
+  int emax = 0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double2 a = G[i];
     const double2 d = dG[i];
     a.x = fma(step, d.x, a.x);
     a.y = fma(step, d.y, a.y);
     G[i] = a;
-    if (G32) G32[i] = make_float2((float)a.x, (float)a.y);
+    if (G32) {
+      G32[i] = make_float2((float)ldexp(a.x, -S), (float)ldexp(a.y, -S));
+      const int e = pair_exp(a.x, a.y);
+      emax = e > emax ? e : emax;
+    }
+  }
+  if (G32) {
+    atomic_max_wave(&F.flags[5], emax);
+    if (blockIdx.x == 0 && threadIdx.x == 0) F.flags[4] = S;
   }
   if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < F.nw; i += blockDim.x) F.v2[i] = fma(step, F.dv2[i], F.v2[i]);
 }
 
-// FP32 shadow of an uploaded state (mixed precision)
-__global__ void k_shadow(const FlowDev *__restrict__ fl) {
+// scale plan for the next shadow write from the exponents recorded since the
+// last plan (one thread per flow, after k_euler / k_maxexp)
+__global__ void k_plan(const FlowDev *__restrict__ fl) {
+  const FlowDev &F = fl[blockIdx.y];
+  if (threadIdx.x != 0 || !F.G32) return;
+  F.flags[6] = shadow_plan(F.flags[5]);
+  F.flags[5] = 0;
+}
+
+// largest binary exponent of G (before the shadow of a state is rebuilt)
+__global__ void k_maxexp(const FlowDev *__restrict__ fl) {
   const FlowDev &F = fl[blockIdx.y];
   if (!F.G32) return;
   const size_t n = (size_t)F.nv;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  int emax = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int e = pair_exp(F.G[i].x, F.G[i].y);
+    emax = e > emax ? e : emax;
+  }
+  atomic_max_wave(&F.flags[5], emax);
+}
+
+// FP32 shadow of the current state with the planned scale (mixed precision)
+__global__ void k_shadow(const FlowDev *__restrict__ fl) {
+  const FlowDev &F = fl[blockIdx.y];
+  if (!F.G32) return;
+  const int S = F.flags[6];
+  const size_t n = (size_t)F.nv;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    F.G32[i] = make_float2((float)F.G[i].x, (float)F.G[i].y);
+    F.G32[i] = make_float2((float)ldexp(F.G[i].x, -S), (float)ldexp(F.G[i].y, -S));
+  if (blockIdx.x == 0 && threadIdx.x == 0) F.flags[4] = S;
 }
 
 // ---------------------------------------------------------------------------
@@ -1206,6 +1276,7 @@ hipError_t launch_euler(hipStream_t st, const FlowDev *d, const FlowDev *h, int 
   if (blocks < 1) blocks = 1;
   ProfScope ps(prof, 3);
   hipLaunchKernelGGL(k_euler, dim3((unsigned)blocks, n), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_plan, dim3(1, n), dim3(64), 0, st, d);
   return hipGetLastError();
 }
 
@@ -1216,6 +1287,20 @@ int upload_fd(sp_ctx *ctx) {
   ctx->h_ring[s] = ctx->fd;
   SP_HIP(hipMemcpyAsync(ctx->d_fd, ctx->h_ring + s, sizeof(FlowDev), hipMemcpyHostToDevice, ctx->stream));
   SP_HIP(hipEventRecord(ctx->ring_ev[s], ctx->stream));
+  return SP_OK;
+}
+
+// FP32 shadow of the current state (mixed form): largest exponent of G ->
+// scale plan -> shadow; a context in the FP64 form keeps its shadow stale
+// until it returns to the mixed form
+int rebuild_shadow(sp_ctx *ctx) {
+  const FlowDev &f = ctx->fd;
+  if (!f.G32 || f.precision != SP_PRECISION_MIXED) return SP_OK;
+  SP_HIP(hipMemsetAsync(f.flags + 5, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_maxexp, dim3(1024, 1), dim3(256), 0, ctx->stream, ctx->d_fd);
+  hipLaunchKernelGGL(k_plan, dim3(1, 1), dim3(64), 0, ctx->stream, ctx->d_fd);
+  hipLaunchKernelGGL(k_shadow, dim3(2048, 1), dim3(256), 0, ctx->stream, ctx->d_fd);
+  SP_HIP(hipGetLastError());
   return SP_OK;
 }
 
@@ -1231,7 +1316,7 @@ int sp_ctx_footprint(const sp_su2_problem *p, long long *bytes) {
   if (p->precision == SP_PRECISION_MIXED) b += nf * p->L * 8;  // FP32 shadow
   b += 3LL * p->nw * 8;                                          // Sigma, dSigma, mesh
   b += 2 * (long long)p->nw * emax * 8 + 2 * emax * 8 + (p->nw + 1) * 4;  // weight tables
-  b += emax * p->L * 16 + 2LL * p->B * p->Rb * 8 + 16;                   // correlation scratch / output, flags
+  b += emax * p->L * 16 + 2LL * p->B * p->Rb * 8 + 32;                   // correlation scratch / output, flags
   b += 4LL * (3 * p->L + 1 + 2 * (p->ovl_off ? p->ovl_off[p->L] : 0) + 2 * p->R + p->B * p->Rb);
   b += 2LL * (p->ovl_off ? p->ovl_off[p->L] : 0);                         // packed overlap indices
   b += (long long)sizeof(FlowDev);
@@ -1265,6 +1350,13 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   int max_lds = 0;
   SP_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
   if (ctx->lds > (size_t)max_lds) return fail(ctx, SP_EARG, "problem too large for the LDS plan of the dGamma kernel");
+  if (f.precision == SP_PRECISION_MIXED) {
+    // a mixed context may switch to the FP64 form (sp_set_precision)
+    int s64, nb64;
+    size_t lds64;
+    launch_shape(f.nw, f.L, 8, s64, nb64, lds64);
+    if (lds64 > (size_t)max_lds) return fail(ctx, SP_EARG, "problem too large for the LDS plan of the dGamma kernel");
+  }
   if (f.nv * 16 >= (1LL << 31)) return fail(ctx, SP_EARG, "vertex too large for 32-bit byte offsets");
   f.g_bytes = (unsigned)(f.nv * 16);
   f.g32_bytes = (unsigned)(f.nv * 8);
@@ -1284,7 +1376,7 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
       (rc = dalloc(ctx, &f.Wc, (size_t)f.nw * emax)) || (rc = dalloc(ctx, &f.Wn, f.nw)) ||
       (rc = dalloc(ctx, &f.Cwp, emax)) || (rc = dalloc(ctx, &f.Cw, emax)) || (rc = dalloc(ctx, &f.Cn, 1)) ||
       (rc = dalloc(ctx, &f.corr_partial, (size_t)emax * f.L * 2)) ||
-      (rc = dalloc(ctx, &f.corr_out, (size_t)2 * f.B * f.Rb + 1)) || (rc = dalloc(ctx, &f.flags, 4)) ||
+      (rc = dalloc(ctx, &f.corr_out, (size_t)2 * f.B * f.Rb + 1)) || (rc = dalloc(ctx, &f.flags, 8)) ||
       (rc = dalloc(ctx, &ctx->d_fd, 1)))
     return rc;
   {
@@ -1335,7 +1427,7 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   if (f.G32) SP_HIP(hipMemset(f.G32, 0, nv * sizeof(float2)));
   SP_HIP(hipMemset(f.v2, 0, f.nw * 8));
   SP_HIP(hipMemset(f.dv2, 0, f.nw * 8));
-  SP_HIP(hipMemset(f.flags, 0, 16));
+  SP_HIP(hipMemset(f.flags, 0, 32));
   SP_HIP(hipMemset(f.Wn, 0, f.nw * 4));
   SP_HIP(hipMemset(f.Cn, 0, 4));
   SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
@@ -1376,14 +1468,26 @@ int sp_set_state(sp_ctx *ctx, double cutoff, const double *v2, const double *ss,
   f.cut = cutoff;
   int rc = upload_fd(ctx);
   if (rc) return rc;
-  if (f.G32) {
-    hipLaunchKernelGGL(k_shadow, dim3(2048, 1), dim3(256), 0, ctx->stream, ctx->d_fd);
-    SP_HIP(hipGetLastError());
-  }
+  if ((rc = rebuild_shadow(ctx))) return rc;
   SP_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_flow = false;
   ctx->flags_cached = false;
   return SP_OK;
+}
+
+int sp_set_precision(sp_ctx *ctx, int precision) {
+  if (!ctx) return SP_EARG;
+  if (precision != SP_PRECISION_FP64 && precision != SP_PRECISION_MIXED)
+    return fail(ctx, SP_EARG, "unsupported precision mode");
+  FlowDev &f = ctx->fd;
+  if (precision == SP_PRECISION_MIXED && !f.G32)
+    return fail(ctx, SP_EARG, "the context holds no FP32 shadow (create it with SP_PRECISION_MIXED)");
+  if (precision == f.precision) return SP_OK;
+  f.precision = precision;
+  launch_shape(f.nw, f.L, precision == SP_PRECISION_MIXED ? 4 : 8, f.slots, f.nb, ctx->lds);
+  int rc = upload_fd(ctx);
+  if (rc) return rc;
+  return rebuild_shadow(ctx);
 }
 
 static int download_pair(sp_ctx *ctx, const double2 *src, double *a, double *b) {

@@ -152,11 +152,23 @@ class FrgCore:
     def measurements(self) -> List[Measurement]:
         return self._measurements
 
-    def takeMeasurements(self) -> None:
-        # non-deferred measurements inside their cutoff window (src/FrgCore.hpp:49-58)
+    def takeMeasurements(self, postprocessing: bool = False, defer_all: bool = False,
+                         data_file: Optional[str] = None) -> None:
+        """``FrgCore::takeMeasurements`` (src/FrgCore.hpp:36-68).  In the flow
+        (postprocessing False): the measurements that are not deferred, inside
+        their cutoff window; then, if any measurement is deferred (or
+        ``defer_all``, the reference's --deferMeasurements), the state is
+        appended to ``data_file`` for the post-processing stage.  In the
+        post-processing stage: the deferred measurements on the state read back
+        from the data file."""
+        deferred = lambda m: defer_all or m.isDeferred()  # noqa: E731
         for m in self._measurements:
-            if m.minCutoff() <= self.cutoff <= m.maxCutoff() and not m.isDeferred():
+            if m.minCutoff() <= self.cutoff <= m.maxCutoff() and deferred(m) == postprocessing:
                 m.takeMeasurement(self)
+        if not postprocessing and any(deferred(m) for m in self._measurements):
+            if data_file is None:
+                raise ArgumentError("deferred measurements need a data file for the post-processing stage")
+            self.writeCheckpoint(data_file, append=True)
 
 
 class HipSU2FrgCore(FrgCore):
@@ -204,9 +216,11 @@ class HipSU2FrgCore(FrgCore):
         _, v2, v4s, v4d = self.context.get_state()
         return write_checkpoint(dataFilePath, self.cutoff, {"v2": v2, "v4dd": v4d, "v4ss": v4s}, append)
 
-    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1) -> bool:
+    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1, missing_ok: bool = False) -> bool:
         """``SU2EffectiveAction::readCheckpoint`` (:159-205); -1 reads the last checkpoint."""
-        from .obs import read_checkpoint
+        from .obs import checkpoint_count, read_checkpoint
+        if missing_ok and not 0 <= checkpointId < checkpoint_count(dataFilePath):
+            return False  # the reference's readCheckpoint returns false past the last checkpoint
         n = self.problem.vertex_size
         cut, a = read_checkpoint(dataFilePath, {"v2": self.problem.nw, "v4dd": n, "v4ss": n}, checkpointId)
         self.setState(cut, a["v2"], a["v4ss"], a["v4dd"])
@@ -260,9 +274,11 @@ class HipTRIFrgCore(FrgCore):
         _, v2, v4 = self.context.get_state()
         return write_checkpoint(dataFilePath, self.cutoff, {"v2": v2, "v4": v4}, append)
 
-    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1) -> bool:
+    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1, missing_ok: bool = False) -> bool:
         """``TRIEffectiveAction::readCheckpoint`` (:159-205)."""
-        from .obs import read_checkpoint
+        from .obs import checkpoint_count, read_checkpoint
+        if missing_ok and not 0 <= checkpointId < checkpoint_count(dataFilePath):
+            return False  # the reference's readCheckpoint returns false past the last checkpoint
         cut, a = read_checkpoint(dataFilePath, {"v2": self.problem.nw, "v4": self.problem.vertex_size}, checkpointId)
         self.setState(cut, a["v2"], a["v4"])
         return True
@@ -287,9 +303,11 @@ class HipXYZFrgCore(HipTRIFrgCore):
         arrays = {"v2": v2, "v4dd": planes[3], "v4xx": planes[0], "v4yy": planes[1], "v4zz": planes[2]}
         return write_checkpoint(dataFilePath, self.cutoff, arrays, append)
 
-    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1) -> bool:
+    def readCheckpoint(self, dataFilePath: str, checkpointId: int = -1, missing_ok: bool = False) -> bool:
         """``XYZEffectiveAction::readCheckpoint`` (:164-217)."""
-        from .obs import read_checkpoint
+        from .obs import checkpoint_count, read_checkpoint
+        if missing_ok and not 0 <= checkpointId < checkpoint_count(dataFilePath):
+            return False  # the reference's readCheckpoint returns false past the last checkpoint
         n = self.problem.vertex_size // 4
         sizes = {"v2": self.problem.nw, "v4dd": n, "v4xx": n, "v4yy": n, "v4zz": n}
         cut, a = read_checkpoint(dataFilePath, sizes, checkpointId)

@@ -57,6 +57,7 @@ def load_obs_library() -> ctypes.CDLL:
                                       ctypes.POINTER(_f32p)]
         lib.sp_ckpt_read.argtypes = [ctypes.c_char_p, ctypes.c_int, _f32p, ctypes.c_int, _cpp, _llp,
                                      ctypes.POINTER(_f32p)]
+        lib.sp_ckpt_count.argtypes = [ctypes.c_char_p]
         _lib = lib
     return _lib
 
@@ -176,3 +177,11 @@ def read_checkpoint(path: str, sizes: dict, checkpoint_id: int = -1):
     if rc < 0:
         raise ObsError(lib.sp_obs_last_error().decode())
     return float(cut.value), dict(zip(names, data))
+
+
+def checkpoint_count(path: str) -> int:
+    """Number of checkpoints in a data file (0 if it does not exist)."""
+    rc = load_obs_library().sp_ckpt_count(path.encode())
+    if rc < 0:
+        raise ObsError(load_obs_library().sp_obs_last_error().decode())
+    return rc

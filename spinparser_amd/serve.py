@@ -20,8 +20,30 @@ from typing import Callable, List, Optional, Tuple
 
 import numpy as np
 
-from .capi import SP_PRECISION_MIXED, FlowContext, FlowGroup, footprint_bytes
+from .capi import SP_PRECISION_FP64, SP_PRECISION_MIXED, FlowContext, FlowGroup, footprint_bytes
 from .problem import SU2Problem
+
+
+def coupling_scale(prob: SU2Problem) -> float:
+    """Largest coupling |J| of the flow (the initial vertex is J / normalization)."""
+    return float(np.abs(prob.coupling_zz_f64).max()) * float(prob.normalization)
+
+
+@dataclass
+class PrecisionSchedule:
+    """Precision form of a mixed-precision flow along its cutoff schedule: the
+    FP64 form for cutoffs at or below ``fp64_below`` times the flow's coupling
+    scale J, the mixed form above.  The state is FP64 in both forms, so the
+    switch only changes how later derivatives are evaluated.  Measured on the
+    J2/J1 sweep, rounding errors of the FP32 integrals made at cutoffs of the
+    order of J and below are amplified by up to three orders of magnitude
+    towards the end of the flow, where the susceptibility of flows close to an
+    ordering instability grows; errors made at large cutoffs are not."""
+
+    fp64_below: float = 3.0
+
+    def precision(self, prob: SU2Problem, cutoff: float) -> int:
+        return SP_PRECISION_FP64 if cutoff <= self.fp64_below * coupling_scale(prob) else SP_PRECISION_MIXED
 
 
 @dataclass
@@ -67,11 +89,16 @@ class _Active:
 class SweepServer:
     """Request loop with footprint admission and continuous batching."""
 
-    def __init__(self, budget_bytes: int, max_flows: int = 64, precision: int = SP_PRECISION_MIXED,
-                 clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep):
+    def __init__(self, budget_bytes: int, max_flows: int = 64, precision: int = SP_PRECISION_FP64,
+                 clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
+                 schedule: Optional[PrecisionSchedule] = None):
         self.budget = int(budget_bytes)
         self.max_flows = int(max_flows)
         self.precision = precision
+        # flows created in the mixed form follow the schedule (None: mixed throughout)
+        self.schedule = schedule if precision == SP_PRECISION_MIXED else None
+        self.fp64_steps = 0
+        self.mixed_steps = 0
         self.clock = clock
         self.sleep = sleep
         self.group = FlowGroup(self.max_flows)
@@ -115,6 +142,16 @@ class SweepServer:
             if log and log_every > 0 and len(self.batch_sizes) % log_every == 0:
                 log("iteration %d: %d active, %d pending, %d done, %.1f GB resident, t=%.1f s" % (
                     len(self.batch_sizes), len(active), len(pending), len(done), used / 1e9, now))
+            for a in active:
+                want = self.precision if self.schedule is None else \
+                    self.schedule.precision(a.prob, float(a.prob.cutoffs[a.k]))
+                if want != a.ctx.precision:
+                    a.ctx.set_precision(want)
+                    a.ctx.synchronize()
+                if want == SP_PRECISION_FP64:
+                    self.fp64_steps += 1
+                else:
+                    self.mixed_steps += 1
             new_cut = [float(a.prob.cutoffs[a.k + 1]) for a in active]
             outs, div = self.group.step([a.ctx for a in active], new_cut)
             finishing: List[_Active] = []
@@ -150,9 +187,11 @@ class SweepServer:
         return sorted(done, key=lambda r: r.ident)
 
 
-def run_alone(req: FlowRequest, precision: int) -> FlowResult:
-    """The same flow driven through the single-flow entry points (no batching)."""
-    from .frgcore import run_core  # noqa: F401  (same loop semantics, restated here for the record lists)
+def run_alone(req: FlowRequest, precision: int, schedule: Optional[PrecisionSchedule] = None,
+              queued: bool = False) -> FlowResult:
+    """The same flow driven through the single-flow entry points (no batching):
+    one ABI call per stage and step, or (queued) the whole cutoff loop through
+    sp_run_flow.  A mixed flow follows the precision schedule when one is given."""
     prob = req.problem
     ctx = FlowContext(prob, precision)
     v2, v4s, v4d = prob.initial_state(np.float64)
@@ -161,7 +200,17 @@ def run_alone(req: FlowRequest, precision: int) -> FlowResult:
     cutoffs, zzs, dds = [], [], []
     k = 0
     diverged = False
-    while k < len(prob.cutoffs) - 1:
+    if queued and schedule is None:
+        recs, diverged = ctx.run_flow(prob.cutoffs)
+        cutoffs = [float(c) for c in prob.cutoffs[:len(recs)]]
+        zzs = [r[0] for r in recs]
+        dds = [r[1] for r in recs]
+        k = len(recs) - 1 if diverged else len(recs)
+    while not diverged and k < len(prob.cutoffs) - 1:
+        if schedule is not None and precision == SP_PRECISION_MIXED:
+            want = schedule.precision(prob, float(prob.cutoffs[k]))
+            if want != ctx.precision:
+                ctx.set_precision(want)
         ctx.compute_step()
         zz, dd = ctx.measure_correlation()
         cutoffs.append(float(prob.cutoffs[k]))

@@ -126,3 +126,33 @@ def test_tri_dm_spin_symmetries():
               "TRICorYY": 1, "TRICorYZ": 1, "TRICorZX": -1, "TRICorZY": 1, "TRICorZZ": 1}
     for name, sgn in parity.items():
         assert np.abs(a[:, g[name]] - sgn * b[:, g[name]]).max() < tol, name
+
+
+def test_kagome_dm_initial_orientation():
+    """The DM term's orientation in the initial TRI vertex, derived by hand from
+    the reference's rules.  kagome-DM (res/models.xml:145-170) writes each bond
+    twice, "xy" with +d and "-yx" with -d (src/LatticeModelFactory.cpp:321-336:
+    strength[s1][s2] += sign * d).  At range 1 the one representative
+    neighbour of the origin (0,0,0,0) is (-1,0,0,1); the model's bond
+    from="-1,0,0,1" to="0,0,0,0" connects it *to* the origin, so
+    isConnectingSites(origin, site) = -1 (src/LatticeModelFactory.cpp:196-207)
+    and the coupling enters transposed (:976-982): J(0 -> rid)[x][y] = -d,
+    J[y][x] = +d.  TRIEffectiveAction sets 0.25 J[s1][s2] / normalization at
+    every frequency (src/TRI/TRIEffectiveAction.hpp:38-64; normalization 1), so
+    with j = 1, d = 0.2: xx = yy = zz = 0.25, xy = -0.05, yx = +0.05, all other
+    components 0, and the on-site vertex (rid 0) is zero."""
+    from helpers import tri_problem_of
+    p = tri_problem_of("kagome", 1, "kagome-DM", {"j": 1.0, "d": 0.2}, 4, 0.05, 10.0, 10.0, 0.1, 0.9)
+    assert p.L == 2
+    assert tuple(p.lattice.sites[1]) == (-1, 0, 0, 1)
+    _, v4 = p.initial_state(np.float64)
+    L = p.L
+    g = v4.reshape(p.n_freq, 16, L)
+    expect = np.zeros((16, L))
+    x, y, z = 0, 1, 2
+    for s1, s2, val in ((x, x, 0.25), (y, y, 0.25), (z, z, 0.25), (x, y, -0.05), (y, x, 0.05)):
+        expect[4 * s1 + s2, 1] = val
+    assert np.allclose(g, expect[None, :, :], rtol=0, atol=1e-7)
+    # the float instantiation carries the same orientation
+    _, v4f = p.initial_state(np.float32)
+    assert np.allclose(v4f.reshape(p.n_freq, 16, L), expect[None, :, :], rtol=0, atol=1e-7)
