@@ -231,16 +231,7 @@ struct TriDev {
 #ifndef SPFRG_TRI_BPAD
 #define SPFRG_TRI_BPAD 6
 #endif
-// T-channel bundles in site-major order (components of one site contiguous,
-// site stride SPFRG_TRI_TSITE doubles); 0 keeps every channel component-major
-#ifndef SPFRG_TRI_TSITE
-#define SPFRG_TRI_TSITE 0
-#endif
-__host__ __device__ inline int tri_bundle_stride(int L) {
-  const int cm = 16 * L + SPFRG_TRI_BPAD;
-  const int sm = SPFRG_TRI_TSITE > 0 ? L * SPFRG_TRI_TSITE + 2 : 0;
-  return cm > sm ? cm : sm;
-}
+__host__ __device__ inline int tri_bundle_stride(int L) { return 16 * L + SPFRG_TRI_BPAD; }
 
 // LDS carve-up of k_tri_dgamma (bytes)
 struct TriLds {
@@ -512,8 +503,6 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
           continue;
         }
         const int d = q / L, j = q - (q / L) * L;  // d = e * 4 + k
-        const int cs = (SPFRG_TRI_TSITE > 0 && ch == 1) ? 1 : L;               // component stride
-        const int ss = (SPFRG_TRI_TSITE > 0 && ch == 1) ? SPFRG_TRI_TSITE : 1;  // site stride
         const int fl = s_fl[d];
         const bool exch = fl & 1, tu = (fl >> 1) & 1;
         const int *cofs = exch ? s_ci : s_cf;
@@ -521,7 +510,7 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
         double w[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) { off[r] = s_off[d * 4 + r]; w[r] = s_w[d * 4 + r]; }
-        double *dst = s_A + (size_t)d * BS + j * ss;
+        double *dst = s_A + (size_t)d * BS + j;
         // four components (16 loads) in flight per group keeps the register
         // footprint of the gather small next to the live accumulators
 #pragma unroll 1
@@ -538,7 +527,7 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
               const double sg = ((fl >> (2 + r)) & 1) ? tsign * fsign : tsign;
               v = fma(sg * w[r], gload(rG, (off[r] + co) * 8), v);
             }
-            dst[cc * cs] = v;
+            dst[cc * L] = v;
           }
         }
       }
@@ -567,30 +556,27 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
           // then adds 2 g o(rr, P2 tt) to out(rr, tt).
           const int gb = F.bgrp_off[cj], ge = F.bgrp_off[cj + 1];
           const double c2 = 2.0 * cw;
-          constexpr bool kSite = SPFRG_TRI_TSITE > 0;
-          const int cs = kSite ? 1 : L, ss = kSite ? SPFRG_TRI_TSITE : 1;
           for (int gi = gb; gi < ge; ++gi) {
             const int4 gr = F.bgrp[gi];
             const int code = gr.z;
             // component rows (4 components, stride L) of A and B
-            const int xr[4] = {4 * (code & 3) * cs, 4 * ((code >> 2) & 3) * cs, 4 * ((code >> 4) & 3) * cs, 12 * cs};
-            const int yr[4] = {4 * ((code >> 6) & 3) * cs, 4 * ((code >> 8) & 3) * cs, 4 * ((code >> 10) & 3) * cs,
-                               12 * cs};
+            const int xr[4] = {4 * (code & 3) * L, 4 * ((code >> 2) & 3) * L, 4 * ((code >> 4) & 3) * L, 12 * L};
+            const int yr[4] = {4 * ((code >> 6) & 3) * L, 4 * ((code >> 8) & 3) * L, 4 * ((code >> 10) & 3) * L, 12 * L};
             double o[16];
 #pragma unroll
             for (int k = 0; k < 16; ++k) o[k] = 0.0;
             for (int i = gr.x; i < gr.y; ++i) {
               const int pk = F.bent[i];
-              const double *xa = X + (pk & 0xffff) * ss;
-              const double *ya = Y + (pk >> 16) * ss;
+              const double *xa = X + (pk & 0xffff);
+              const double *ya = Y + (pk >> 16);
               double bv[16];
 #pragma unroll
               for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-                for (int cc = 0; cc < 4; ++cc) bv[4 * mm + cc] = ya[yr[mm] + cc * cs];
+                for (int cc = 0; cc < 4; ++cc) bv[4 * mm + cc] = ya[yr[mm] + cc * L];
 #pragma unroll
               for (int rr = 0; rr < 4; ++rr) {
-                const double av[4] = {xa[xr[rr]], xa[xr[rr] + cs], xa[xr[rr] + 2 * cs], xa[xr[rr] + 3 * cs]};
+                const double av[4] = {xa[xr[rr]], xa[xr[rr] + L], xa[xr[rr] + 2 * L], xa[xr[rr] + 3 * L]};
 #pragma unroll
                 for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
@@ -611,12 +597,12 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
           // chalice (pair 0: bundle 0 with local 4; pair 1: bundle 2 with local 5)
           const double *lc = s_loc + (ce * 4 + cpair) * 16;
 #pragma unroll
-          for (int k = 0; k < 16; ++k) { x[k] = cw * X[k * cs + cj * ss]; y[k] = lc[k]; }
+          for (int k = 0; k < 16; ++k) { x[k] = cw * X[k * L + cj]; y[k] = lc[k]; }
           apply_table<2>(acc, x, y);
           // inverse chalice (pair 0: local 6 with bundle 1; pair 1: local 7 with bundle 3)
           const double *li = s_loc + (ce * 4 + 2 + cpair) * 16;
 #pragma unroll
-          for (int k = 0; k < 16; ++k) { x[k] = cw * li[k]; y[k] = Y[k * cs + cj * ss]; }
+          for (int k = 0; k < 16; ++k) { x[k] = cw * li[k]; y[k] = Y[k * L + cj]; }
           apply_table<3>(acc, x, y);
         }
       }
