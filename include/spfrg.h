@@ -147,6 +147,30 @@ int sp_profile_get(sp_ctx *ctx, int kernel, double *total_ms, long *launches);
 /* Device bytes held by the context (admission by footprint). */
 int sp_ctx_footprint(const sp_su2_problem *problem, long long *bytes);
 
+/* The FP64 computeStep without allocating: for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  No counterpart in the reference.  The caller owns the
+ * state and the flow on the device:
+ *   d_v2[Nw] (Sigma) and d_v4[2 * Nf * L] (Gamma) in, d_dv2[Nw] and
+ *   d_dv4[2 * Nf * L] out, Gamma and dGamma interleaved as {spin, density}
+ *   pairs at the reference index (Gamma[2 i] = spin, Gamma[2 i + 1] = density
+ *   of entry i, i = (s(s+1)/2 + u) * Nw * L + t * L + rid), Nf = Nw^2 (Nw+1)/2.
+ * problem->precision must be SP_PRECISION_FP64.  Descriptor, flags, lattice
+ * tables and weight tables live in one library-owned device array of
+ * SP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends); problems
+ * whose tables exceed it are rejected with SP_EARG (N_w = 64 with 8,000
+ * overlap entries needs about 0.2 MB).  The call waits for earlier work on
+ * `stream` (a hipStream_t; null = the null stream), uploads the tables
+ * synchronously and queues dSigma and dGamma on `stream`.  diverged non-null:
+ * the call also waits for the result and reports a NaN in the flow
+ * (SpinParser::runCore's isDiverged check); null: fully asynchronous after
+ * the upload.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other threads, other streams) race.  Results equal
+ * sp_compute_step on a context holding the same state. */
+#define SP_NOALLOC_SCRATCH_BYTES (4 << 20)
+int sp_compute_step_noalloc(const sp_su2_problem *problem, double cutoff, const double *d_v2, const double *d_v4,
+                            double *d_dv2, double *d_dv4, void *stream, int *diverged);
+
 /* Flow groups (no reference counterpart; replaces running several SpinParser
  * processes side by side).  A group batches resident flows of any sizes into
  * one launch per kernel and step.  sp_group_step performs, for every flow i,

@@ -27,7 +27,7 @@ EXPORTED_SYMBOLS = (
     "sp_ctx_create", "sp_ctx_destroy", "sp_set_state", "sp_get_state", "sp_get_flow", "sp_compute_step",
     "sp_finalize_step", "sp_is_diverged", "sp_measure_correlation", "sp_run_flow", "sp_synchronize", "sp_last_error",
     "sp_profile_enable", "sp_profile_get", "sp_ctx_footprint", "sp_group_create", "sp_group_destroy",
-    "sp_group_step", "sp_group_measure", "sp_group_last_error", "sp_set_precision",
+    "sp_group_step", "sp_group_measure", "sp_group_last_error", "sp_set_precision", "sp_compute_step_noalloc",
 )
 
 

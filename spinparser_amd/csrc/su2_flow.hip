@@ -55,6 +55,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -1838,3 +1839,171 @@ int sp_group_measure(sp_group *grp, sp_ctx *const *ctxs, int n, double *const *z
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// sp_compute_step_noalloc: the FP64 computeStep on caller-owned device buffers.
+// Everything else the kernels need -- descriptor, flags, mesh, lattice tables
+// and the per-step weight tables -- lives in one file-scope device array of
+// SP_NOALLOC_SCRATCH_BYTES, so the call allocates nothing.  Each device holds
+// its own copy of the array; its address is looked up once per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[SP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char *g_noalloc_ptr[kNoAllocDevices] = {};  // device address of scratch, per device
+int g_noalloc_lds[kNoAllocDevices] = {};            // LDS limit per workgroup, per device
+unsigned char g_noalloc_image[SP_NOALLOC_SCRATCH_BYTES];  // host image of the scratch contents
+
+struct NoAllocPlan {
+  size_t fd, flags, mesh, site, inv, ooff, o1, o2, oidx, rf, ri, wwp, wc, wn, cwp, cw, cn, total;
+};
+
+NoAllocPlan noalloc_plan(const sp_su2_problem *p) {
+  NoAllocPlan q{};
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = (o + bytes + 255) & ~size_t(255);
+    return at;
+  };
+  const size_t O = (size_t)p->ovl_off[p->L];
+  const size_t emax = 2 * (size_t)p->nw + 8;
+  q.fd = take(sizeof(FlowDev));
+  q.flags = take(8 * sizeof(int));
+  q.mesh = take((size_t)p->nw * 8);
+  q.site = take((size_t)p->L * 4);
+  q.inv = take((size_t)p->L * 4);
+  q.ooff = take(((size_t)p->L + 1) * 4);
+  q.o1 = take(O * 4);
+  q.o2 = take(O * 4);
+  q.oidx = take(O * 2);
+  q.rf = take((size_t)p->R * 4);
+  q.ri = take((size_t)p->R * 4);
+  q.wwp = take((size_t)p->nw * emax * 8);
+  q.wc = take((size_t)p->nw * emax * 8);
+  q.wn = take((size_t)p->nw * 4);
+  q.cwp = take(emax * 8);
+  q.cw = take(emax * 8);
+  q.cn = take(4);
+  q.total = o;
+  return q;
+}
+
+// device address of this device's scratch array (looked up once per device
+// with the device current); callers hold g_noalloc_mu
+int noalloc_scratch(int dev, unsigned char **base, int *max_lds) {
+  if (dev < 0 || dev >= kNoAllocDevices) return SP_EARG;
+  if (!g_noalloc_ptr[dev]) {
+    void *q = nullptr;
+    if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return SP_EHIP;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return SP_EHIP;
+    if (hipFuncSetAttribute((const void *)k_dgamma<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) !=
+        hipSuccess)
+      return SP_EHIP;
+    g_noalloc_lds[dev] = lds;
+    g_noalloc_ptr[dev] = static_cast<unsigned char *>(q);
+  }
+  *base = g_noalloc_ptr[dev];
+  *max_lds = g_noalloc_lds[dev];
+  return SP_OK;
+}
+
+}  // namespace
+
+extern "C" int sp_compute_step_noalloc(const sp_su2_problem *p, double cutoff, const double *d_v2,
+                                       const double *d_v4, double *d_dv2, double *d_dv4, void *stream,
+                                       int *diverged) {
+  if (check_problem(nullptr, p) != SP_OK) return SP_EARG;
+  if (p->precision != SP_PRECISION_FP64) return SP_EARG;
+  if (!d_v2 || !d_v4 || !d_dv2 || !d_dv4 || !std::isfinite(cutoff) || cutoff <= 0.0) return SP_EARG;
+  const NoAllocPlan q = noalloc_plan(p);
+  if (q.total > (size_t)SP_NOALLOC_SCRATCH_BYTES) return SP_EARG;
+  FlowDev f{};
+  f.nw = p->nw;
+  f.L = p->L;
+  f.R = p->R;
+  f.B = p->B;
+  f.Rb = p->Rb;
+  f.nf = (long long)p->nw * p->nw * (p->nw + 1) / 2;
+  f.nv = f.nf * p->L;
+  if (f.nv * 16 >= (1LL << 31)) return SP_EARG;
+  f.emax = 2 * p->nw + 8;
+  f.spin = p->spin;
+  f.precision = SP_PRECISION_FP64;
+  f.g_bytes = (unsigned)(f.nv * 16);
+  f.cut = cutoff;
+  size_t lds = 0;
+  launch_shape(f.nw, f.L, 8, f.slots, f.nb, lds);
+  double mesh_d[256];
+  for (int i = 0; i < f.nw; ++i) mesh_d[i] = (double)p->mesh[i];
+  geo_mesh_params(mesh_d, f.nw, f.geo, f.ga, f.gb);
+
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return SP_EHIP;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  unsigned char *base = nullptr;
+  int max_lds = 0;
+  int rc = noalloc_scratch(dev, &base, &max_lds);
+  if (rc != SP_OK) return rc;
+  if (lds > (size_t)max_lds) return SP_EARG;
+  // an earlier call on this stream may still read the scratch
+  if (hipStreamSynchronize(st) != hipSuccess) return SP_EHIP;
+
+  unsigned char *h = g_noalloc_image;
+  std::memset(h, 0, q.total);  // zero flags and counters
+  const int O = p->ovl_off[p->L];
+  const int Lp = (f.L + 3) & ~3;
+  std::memcpy(h + q.mesh, mesh_d, (size_t)f.nw * 8);
+  std::memcpy(h + q.site, p->site_rid, (size_t)f.L * 4);
+  std::memcpy(h + q.inv, p->inv_rid, (size_t)f.L * 4);
+  std::memcpy(h + q.ooff, p->ovl_off, ((size_t)f.L + 1) * 4);
+  if (O > 0) {
+    std::memcpy(h + q.o1, p->ovl_rid1, (size_t)O * 4);
+    std::memcpy(h + q.o2, p->ovl_rid2, (size_t)O * 4);
+    unsigned short *idx = reinterpret_cast<unsigned short *>(h + q.oidx);
+    for (int i = 0; i < O; ++i) idx[i] = (unsigned short)(p->ovl_rid1[i] * Lp + p->ovl_rid2[i]);
+  }
+  std::memcpy(h + q.rf, p->range_fwd, (size_t)f.R * 4);
+  std::memcpy(h + q.ri, p->range_inv, (size_t)f.R * 4);
+  f.G = reinterpret_cast<double2 *>(const_cast<double *>(d_v4));
+  f.dG = reinterpret_cast<double2 *>(d_dv4);
+  f.G32 = nullptr;
+  f.v2 = const_cast<double *>(d_v2);
+  f.dv2 = d_dv2;
+  f.mesh = reinterpret_cast<const double *>(base + q.mesh);
+  f.site_rid = reinterpret_cast<const int *>(base + q.site);
+  f.inv_rid = reinterpret_cast<const int *>(base + q.inv);
+  f.ovl_off = reinterpret_cast<const int *>(base + q.ooff);
+  f.ovl_rid1 = reinterpret_cast<const int *>(base + q.o1);
+  f.ovl_rid2 = reinterpret_cast<const int *>(base + q.o2);
+  f.ovl_idx = reinterpret_cast<const unsigned short *>(base + q.oidx);
+  f.range_fwd = reinterpret_cast<const int *>(base + q.rf);
+  f.range_inv = reinterpret_cast<const int *>(base + q.ri);
+  f.corr_rid = nullptr;
+  f.Wwp = reinterpret_cast<double *>(base + q.wwp);
+  f.Wc = reinterpret_cast<double *>(base + q.wc);
+  f.Wn = reinterpret_cast<int *>(base + q.wn);
+  f.Cwp = reinterpret_cast<double *>(base + q.cwp);
+  f.Cw = reinterpret_cast<double *>(base + q.cw);
+  f.Cn = reinterpret_cast<int *>(base + q.cn);
+  f.corr_partial = nullptr;
+  f.corr_out = nullptr;
+  f.flags = reinterpret_cast<int *>(base + q.flags);
+  std::memcpy(h + q.fd, &f, sizeof(FlowDev));
+  if (hipMemcpy(base, h, q.total, hipMemcpyHostToDevice) != hipSuccess) return SP_EHIP;
+  if (launch_compute(st, reinterpret_cast<const FlowDev *>(base + q.fd), &f, 1, nullptr) != hipSuccess)
+    return SP_EHIP;
+  if (diverged) {
+    int fl[2] = {0, 0};
+    if (hipStreamSynchronize(st) != hipSuccess) return SP_EHIP;
+    if (hipMemcpy(fl, base + q.flags, sizeof(fl), hipMemcpyDeviceToHost) != hipSuccess) return SP_EHIP;
+    if (fl[1]) return SP_EARG;  // evaluation list overflow
+    *diverged = fl[0] ? 1 : 0;
+  }
+  return SP_OK;
+}
