@@ -828,13 +828,18 @@ __device__ __forceinline__ int pair_exp(double a, double b) {
   const int eb = (int)((__double_as_longlong(b) >> 52) & 0x7ff);
   return (ea > eb ? ea : eb) - 1023;
 }
+// wave maximum of v into *dst.  Exponents up to kShadowExp plan no scaling,
+// so only larger ones are recorded, and only when they exceed the value
+// already there: a flow of ordinary magnitude issues no atomic at all, where
+// one atomic per wave on the one flag address serialized the Euler update
+// (about 1 ms per step at 3 M vertex entries)
 __device__ __forceinline__ void atomic_max_wave(int *dst, int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const int w = __shfl_xor(v, o);
     v = w > v ? w : v;
   }
-  if ((threadIdx.x & 63) == 0) atomicMax(dst, v);
+  if ((threadIdx.x & 63) == 0 && v > kShadowExp && v > __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMax(dst, v);
 }
 
 // in-place explicit Euler step (src/SU2/SU2FrgCore.cpp:111-137); a flow whose

@@ -228,6 +228,11 @@ struct TriDev {
 // off a common bank: the lattice bubble's lanes read the same site of
 // different bundles at once, and a stride that is a multiple of 16 doubles
 // puts every bundle on the same banks (measured: 1 % faster at L = 19)
+// lattice bubble: load the next overlap entry's packed sites ahead (1) or
+// right before use (0)
+#ifndef SPFRG_TRI_PREFETCH
+#define SPFRG_TRI_PREFETCH 1
+#endif
 #ifndef SPFRG_TRI_BPAD
 #define SPFRG_TRI_BPAD 6
 #endif
@@ -565,8 +570,13 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
             double o[16];
 #pragma unroll
             for (int k = 0; k < 16; ++k) o[k] = 0.0;
+            // the next entry's packed sites are loaded while the current
+            // entry is evaluated (the load is dependent: its result forms
+            // the LDS addresses of the next 32 reads)
+            int pk_next = (SPFRG_TRI_PREFETCH && gr.x < gr.y) ? F.bent[gr.x] : 0;
             for (int i = gr.x; i < gr.y; ++i) {
-              const int pk = F.bent[i];
+              const int pk = SPFRG_TRI_PREFETCH ? pk_next : F.bent[i];
+              if (SPFRG_TRI_PREFETCH && i + 1 < gr.y) pk_next = F.bent[i + 1];
               const double *xa = X + (pk & 0xffff);
               const double *ya = Y + (pk >> 16);
               double bv[16];
