@@ -376,11 +376,18 @@ __device__ __forceinline__ int tri_support(int so, int to, int uo, int nw, int F
 __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev F) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const long long total = F.nf;
+  // workgroups are dispatched round-robin over the 8 XCDs: each XCD receives
+  // runs of kTriChunk consecutive positions, and positions walk the items
+  // t-major (every (s,u) row of one t before the next t), so the workgroups
+  // resident on an XCD share the t-slab that all T-channel bundles read
+  // (N_w (N_w + 1) / 2 support rows, 2.9 MB at L = 19, N_w = 48)
   const long long j0 = blockIdx.x >> 3;
-  const long long item = ((j0 / kTriChunk) * 8 + (blockIdx.x & 7)) * kTriChunk + (j0 % kTriChunk);
-  if (item >= total) return;
+  const long long pos = ((j0 / kTriChunk) * 8 + (blockIdx.x & 7)) * kTriChunk + (j0 % kTriChunk);
+  if (pos >= total) return;
 
   const int nw = F.nw, L = F.L, nb = F.nb;
+  const long long nsu = total / nw;
+  const long long item = (pos % nsu) * nw + pos / nsu;
   const int FS = 16 * L;
   const int BS = tri_bundle_stride(L);
   const TriLds lay = tri_lds(nw, L, nb);
