@@ -68,6 +68,16 @@ def max_rel(a, b):
     return num / den
 
 
+def elem_rel(a, b, floor=1e-4):
+    """Elementwise relative error max_i |a_i - b_i| / max(|b_i|, floor * max|b|):
+    entries below the floor are held to the bar in units of the floor."""
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    top = np.abs(b).max()
+    if top == 0.0:
+        return float(np.abs(a).max())
+    return float((np.abs(a - b) / np.maximum(np.abs(b), floor * top)).max())
+
+
 def tri_ref3_problem():
     """TRI honeycomb range 3, Heisenberg-Kitaev-Gamma j=0.2 k=1 g=-0.1 (test/scripted/test_reference3.sh:17-60)."""
     from spinparser_amd.problem import tri_problem

@@ -6,12 +6,14 @@ against the FP64 CPU oracle.
   states the GPU flow reached, and the whole FP64 correlation flow against the
   oracle's whole FP64 flow;
 * mixed precision on simple cubic J1 = 1, J2 = 0.3, range 6, 64 frequencies in
-  [0.002, 100]: sampled dGamma items at three cutoffs against the FP64 oracle
-  (dSigma is FP64 in both forms), and the whole mixed correlation flow against
-  the FP64 GPU flow;
+  [0.002, 100]: sampled dGamma items at three cutoffs and every item at the
+  middle one against the FP64 oracle (dSigma is FP64 in both forms), and the
+  whole mixed correlation flow against the FP64 GPU flow;
 * TRI kagome J = 1, D = 0.2 out of plane, range 5, 48 frequencies in
-  [0.005, 50]: sampled dGamma items, dSigma and all ten correlations at three
-  cutoffs on states the GPU flow reached;
+  [0.005, 50], on states the GPU flow reached: every dGamma item and dSigma at
+  the first cutoff and a quarter into the schedule against the oracle's full
+  evaluation; sampled dGamma items, dSigma and all ten correlations at three
+  cutoffs;
 * core options: SU(2) with spin S = 1 and with a normalization, TRI with a
   normalization.
 
@@ -27,7 +29,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import cfg1_problem, cfg2_problem, cfg4_problem, exp_problem, max_rel, tri_problem_of
+from helpers import cfg1_problem, cfg2_problem, cfg4_problem, elem_rel, exp_problem, max_rel, tri_problem_of
 
 pytestmark = pytest.mark.gpu
 
@@ -40,20 +42,16 @@ TOL_MIXED = 1e-6
 # floor (1e-14 of the largest entry).  The correlations keep a floor of 1e-6.
 FLOOR = 1e-4
 FLOOR_CORR = 1e-6
+# the whole-array checks meet the worst entries of millions: the terms of an
+# entry can be far larger than the entry, so entries carry an absolute rounding
+# error of up to a few 1e-14 of the array's largest entry; entries below 1e-3
+# of it are held to 1e-13 of it
+FLOOR_ALL = 1e-3
 
 
 def _log(msg):
     print(msg, flush=True)
     sys.stdout.flush()
-
-
-def elem_rel(a, b, floor=FLOOR):
-    """max_i |a_i - b_i| / max(|b_i|, floor * max|b|)."""
-    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
-    top = np.abs(b).max()
-    if top == 0.0:
-        return float(np.abs(a).max())
-    return float((np.abs(a - b) / np.maximum(np.abs(b), floor * top)).max())
 
 
 def _check(name, got, ref, tol, elementwise=True, floor=FLOOR):
@@ -167,6 +165,11 @@ def test_cubic_r6_nw64_mixed_vs_oracle():
         worst["dGamma density"] = max(worst["dGamma density"],
                                       _check("dGamma density", np.concatenate(gd), np.concatenate(rd), TOL_MIXED,
                                              elementwise=False))
+        if k == ks[1]:
+            # every item (6.1 million values per array) against the oracle's full evaluation
+            _, fs, fd, _ = o.compute_step(cut, v2, v4s, v4d)
+            _check("dGamma spin (all)", dss, fs, TOL_MIXED, elementwise=False)
+            _check("dGamma density (all)", ddd, fd, TOL_MIXED, elementwise=False)
     ctx.close()
     _log("  mixed vs FP64 oracle, max over 3 cutoffs x 64 items: %s" % worst)
     # the whole correlation flow, mixed against FP64 (655 records)
@@ -194,12 +197,21 @@ def test_kagome_dm_r5_nw48_sampled():
     o = TriOracle(p)
     n = len(p.cutoffs) - 1
     ks = [0, n // 2, n - 1]
+    full = {0, n // 4}  # every dGamma item (17.2 million values) against the oracle's full evaluation
     ctx = TriFlowContext(p)
     ctx.set_state(float(p.cutoffs[0]), *p.initial_state(np.float64))
     rng = np.random.default_rng(5)
     L16 = 16 * p.L
     for k in range(n):
         ctx.compute_step()
+        if k in full:
+            cut = float(p.cutoffs[k])
+            _, v2, v4 = ctx.get_state()
+            dv2, dv4 = ctx.get_flow()
+            r2, r4, _ = o.compute_step(cut, v2, v4)
+            _log("  k=%d cutoff %.5g, all items" % (k, cut))
+            _check("dSigma", dv2, r2, TOL)
+            _check("dGamma (all)", dv4, r4, TOL, floor=FLOOR_ALL)
         if k in ks:
             cut = float(p.cutoffs[k])
             _log("  k=%d cutoff %.5g" % (k, cut))

@@ -9,12 +9,13 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import cfg1_problem, cfg2_problem, max_rel, ref1_problem, python_obs_problem
+from helpers import cfg1_problem, cfg2_problem, elem_rel, max_rel, ref1_problem, python_obs_problem
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
+FLOOR_ALL = 1e-3  # elementwise floor of the whole-array checks (fraction of the largest entry)
 
 
 def _log(msg):
@@ -139,9 +140,16 @@ def test_divergence_flag_from_host_nan():
 
 
 def test_cfg2_sampled_items():
-    """Simple cubic J1-J2 range 6, N_w = 64 (the benchmark flow): dSigma and a
-    sample of dGamma items at three cutoffs against the oracle, on states the GPU
-    flow itself reached."""
+    """Simple cubic J1-J2 range 6, N_w = 64 (the benchmark flow) on states the
+    GPU flow itself reached: dSigma and a sample of 96 dGamma items at the
+    first cutoff, and dSigma and every one of the 133,120 items (6.1 million
+    values per array) at cutoffs 300 and 560 against the oracle's full
+    evaluation, normwise and elementwise with a floor of 1e-3 of the array's
+    largest entry.  The GPU reassociates the sums (lattice bubble as rank
+    updates, reduction order), and the terms of an entry can be far larger than
+    the entry, so every entry carries an absolute rounding error of up to a few
+    1e-14 of the array's largest entry (4.2e-14 at cutoff 560); entries below
+    the floor are held to 1e-13 of the largest entry."""
     prob = cfg2_problem()
     _log("cfg2 L=%d nw=%d nf=%d" % (prob.L, prob.nw, prob.n_freq))
     o = Oracle(prob)
@@ -149,25 +157,34 @@ def test_cfg2_sampled_items():
     v2, v4s, v4d = prob.initial_state(np.float64)
     ctx.set_state(float(prob.cutoffs[0]), v2, v4s, v4d)
     rng = np.random.default_rng(7)
-    check_at = {0, 300, 560}
+    sampled, full = {0}, {300, 560}
     L = prob.L
     for k in range(561):
         ctx.compute_step()
-        if k in check_at:
+        if k in sampled | full:
             cut = float(prob.cutoffs[k])
             _, s2, s4s, s4d = ctx.get_state()
             dv2, dss, ddd = ctx.get_flow()
-            dv2_ref = o.dsigma(cut, s2, s4s, s4d)
-            items = np.sort(rng.choice(prob.n_freq, 96, replace=False))
-            got, ref = [], []
-            for it in items:
-                rs, rd, _ = o.items(cut, s2, dv2_ref, s4s, s4d, int(it), int(it) + 1)
-                sl = slice(it * L, (it + 1) * L)
-                got += [dss[sl], ddd[sl]]
-                ref += [rs[sl], rd[sl]]
-            e_sig = max_rel(dv2, dv2_ref)
-            e_items = max_rel(np.concatenate(got), np.concatenate(ref))
-            _log("  k=%d cutoff=%.5g dSigma err %.3g sampled dGamma err %.3g" % (k, cut, e_sig, e_items))
-            assert e_sig < TOL and e_items < TOL
+            if k in full:
+                dv2_ref, rs, rd, _ = o.compute_step(cut, s2, s4s, s4d)
+                e_sig = max_rel(dv2, dv2_ref)
+                e_s, e_d = max_rel(dss, rs), max_rel(ddd, rd)
+                el_s, el_d = elem_rel(dss, rs, FLOOR_ALL), elem_rel(ddd, rd, FLOOR_ALL)
+                _log("  k=%d cutoff=%.5g dSigma err %.3g, all items: spin %.3g (elementwise %.3g), "
+                     "density %.3g (elementwise %.3g)" % (k, cut, e_sig, e_s, el_s, e_d, el_d))
+                assert max(e_sig, e_s, e_d, el_s, el_d) < TOL
+            else:
+                dv2_ref = o.dsigma(cut, s2, s4s, s4d)
+                items = np.sort(rng.choice(prob.n_freq, 96, replace=False))
+                got, ref = [], []
+                for it in items:
+                    rs, rd, _ = o.items(cut, s2, dv2_ref, s4s, s4d, int(it), int(it) + 1)
+                    sl = slice(it * L, (it + 1) * L)
+                    got += [dss[sl], ddd[sl]]
+                    ref += [rs[sl], rd[sl]]
+                e_sig = max_rel(dv2, dv2_ref)
+                e_items = max_rel(np.concatenate(got), np.concatenate(ref))
+                _log("  k=%d cutoff=%.5g dSigma err %.3g sampled dGamma err %.3g" % (k, cut, e_sig, e_items))
+                assert e_sig < TOL and e_items < TOL
         ctx.finalize_step(float(prob.cutoffs[k + 1]))
     ctx.close()
