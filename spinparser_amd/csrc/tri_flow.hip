@@ -22,7 +22,7 @@
 // by hand.  Each table expands into straight-line FMAs on register arrays.
 //
 // dGamma kernel: one workgroup of 256 threads per frequency triple (s, t, u),
-// XCD-chunked like the SU(2) kernel.  Per channel the evaluation points are
+// items walked t-major in XCD runs.  Per channel the evaluation points are
 // processed in chunks of nb points:
 //   A  one thread per (point, access buffer) builds the descriptor (4 supports,
 //      weights, exchange / sign flags), the T channel's four site-local
@@ -40,7 +40,6 @@
 // The per-thread sums are reduced through LDS at the end of the item.
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
