@@ -4,8 +4,11 @@
 //   G, dG   vertex and its flow, interleaved channels: double2 {spin, density}
 //           at [(tri(s,u) * Nw + t) * L + rid], tri(s,u) = s(s+1)/2 + u, s >= u.
 //           One 16-byte load fetches both channels of a support point.
-//   G32     FP32 shadow of G (float2, same index) for the mixed-precision form;
-//           rewritten by every Euler step from the FP64 master copy.
+//   G32     FP32 shadow of G (float2, same index) for the mixed-precision form,
+//           G * 2^-S with a power-of-two scale S planned from the largest
+//           exponent of G; rewritten by every Euler step from the FP64 master
+//           copy.  Every dGamma term is quadratic in G, so the FP32 result is
+//           rescaled by 2^(2S).
 //   v2,dv2  self-energy and its flow, Nw doubles.
 //   W       per exact-frequency index x: the evaluation points w' of the
 //           frequency integral of a channel whose transfer frequency is
