@@ -441,6 +441,16 @@ def sample_indices(nsteps_flow, steps):
     return [min(nsteps_flow - 1, int((i + 0.5) * nsteps_flow / steps)) for i in range(steps)]
 
 
+def full_schedule_seconds(step_flops, ks, t_steps):
+    """Whole-schedule time from timed samples: the seconds per algorithmic FLOP
+    measured at the sampled evaluation indices `ks`, interpolated linearly in
+    the index (held constant beyond the first and last sample), times the FLOP
+    of every evaluation of the schedule."""
+    step_flops = np.asarray(step_flops, dtype=np.float64)
+    spf = np.array([t / step_flops[k] for t, k in zip(t_steps, ks)])
+    return float(np.sum(step_flops * np.interp(np.arange(len(step_flops)), ks, spf)))
+
+
 def stratified_leg(prob, precision, steps, warmup, label):
     """Exactly `steps` iterations of the flow loop (computeStep, correlation,
     isDiverged, finalizeStep; one ABI call per stage) at cutoffs stratified
@@ -494,8 +504,7 @@ def stratified_leg(prob, precision, steps, warmup, label):
         dg_n += n
         c.close()
     print("# %s: %d stratified steps %.3f s" % (label, steps, elapsed), file=sys.stderr, flush=True)
-    spf = np.array([t / step_flops[k] for t, k in zip(t_steps, ks)])
-    est = float(np.sum(np.array(step_flops) * np.interp(np.arange(nsteps_flow), ks, spf)))
+    est = full_schedule_seconds(step_flops, ks, t_steps)
     return dict(elapsed=elapsed, dg_ms=dg_ms, dg_n=dg_n, flops=float(sum(step_flops[k] for k in ks)),
                 est_full_s=est, indices=ks)
 

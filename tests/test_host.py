@@ -161,3 +161,36 @@ def test_invalid_problem_rejected_without_gpu_work():
     from spinparser_amd.capi import FlowError, footprint_bytes
     prob = cfg1_problem(nw=8)
     assert footprint_bytes(prob) > 2 * prob.vertex_size * 16
+
+
+def test_bench_stratified_sampling_and_estimator():
+    """The bench line's sampled rate: strata midpoints over the schedule, and
+    the whole-schedule time from seconds per algorithmic FLOP interpolated
+    between the samples (exact when that rate is constant or linear in the
+    evaluation index)."""
+    import bench
+    n = 655
+    ks = bench.sample_indices(n, 20)
+    assert len(ks) == 20 and ks == sorted(set(ks)) and ks[0] >= 0 and ks[-1] < n
+    assert ks[0] < n / 20 and ks[-1] >= n - n / 20
+    assert bench.sample_indices(n, n) == list(range(n))
+    rng = np.random.default_rng(1)
+    flops = rng.uniform(1e9, 4e11, n)
+    spf = 2.5e-14
+    exact = spf * flops.sum()
+    assert abs(bench.full_schedule_seconds(flops, ks, [spf * flops[k] for k in ks]) - exact) <= 1e-12 * exact
+    a, b = 1e-14, 3e-17
+    t = [(a + b * k) * flops[k] for k in ks]
+    want = sum(flops[i] * (a + b * min(max(i, ks[0]), ks[-1])) for i in range(n))
+    assert abs(bench.full_schedule_seconds(flops, ks, t) - want) <= 1e-9 * want
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message before any GPU work."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
