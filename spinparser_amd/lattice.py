@@ -220,17 +220,17 @@ class LatticeUnitCell:
                 if uc.tag != "unitcell":
                     continue
                 if uc.get("name") is None:
-                    raise LatticeError("Invalid unit cell. Unit cell name is undefined.")
+                    raise LatticeError("lattice unit cell without a name attribute")
                 if uc.get("name") != name:
                     continue
                 out = cls()
                 prims = [c for c in uc if c.tag == "primitive"]
                 if len(prims) != 3:
-                    raise LatticeError("Invalid unit cell. Unit cell must define three primitive vectors.")
+                    raise LatticeError("unit cell '%s' needs exactly three primitive vectors" % name)
                 if not any(c.tag == "site" for c in uc):
-                    raise LatticeError("Invalid unit cell. Unit cell must contain at least one lattice site.")
+                    raise LatticeError("unit cell '%s' has no lattice site" % name)
                 if not any(c.tag == "bond" for c in uc):
-                    raise LatticeError("Invalid unit cell. Unit cell must contain at least one lattice bond.")
+                    raise LatticeError("unit cell '%s' has no lattice bond" % name)
                 for c in uc:
                     if c.tag in ("primitive", "site"):
                         v = (string_to_double(c.get("x")), string_to_double(c.get("y")), string_to_double(c.get("z")))
@@ -239,7 +239,7 @@ class LatticeUnitCell:
                         out.bonds.append((int(c.get("from")), int(c.get("to")), int(c.get("da0")),
                                           int(c.get("da1")), int(c.get("da2"))))
                 return out
-        raise LatticeError("Invalid lattice unit cell definition.")
+        raise LatticeError("no lattice unit cell named '%s'" % name)
 
 
 @dataclass
@@ -294,23 +294,23 @@ class SpinModelUnitCell:
                 if model.tag != "model":
                     continue
                 if model.get("name") is None:
-                    raise LatticeError("Invalid spin model. Model name is undefined.")
+                    raise LatticeError("spin model without a name attribute")
                 if model.get("name") != name:
                     continue
                 out = cls()
                 if not any(c.tag == "interaction" for c in model):
-                    raise LatticeError("Invalid spin model. Spin model must define at least one interaction.")
+                    raise LatticeError("spin model '%s' has no interaction" % name)
                 for c in model:
                     if c.tag != "interaction":
                         continue
                     param, frm, to, typ = c.get("parameter"), c.get("from"), c.get("to"), c.get("type")
                     if None in (param, frm, to, typ):
-                        raise LatticeError("Invalid interaction definition.")
+                        raise LatticeError("an interaction of spin model '%s' lacks its parameter, from, to or type" % name)
                     m1, m2 = _RE_SITE.fullmatch(frm), _RE_SITE.fullmatch(to)
                     if not m1 or not m2:
-                        raise LatticeError("Invalid spin model. Site is ill-defined.")
+                        raise LatticeError("interaction sites '%s' / '%s' of spin model '%s' cannot be parsed" % (frm, to, name))
                     if param not in options:
-                        raise LatticeError("Interaction parameter '%s' is not defined in the taskfile" % param)
+                        raise LatticeError("the task file sets no value for interaction parameter '%s'" % param)
                     val = string_to_float(options[param])
                     st = np.zeros((3, 3), dtype=np.float32)
                     if typ == "heisenberg":
@@ -326,14 +326,14 @@ class SpinModelUnitCell:
                     else:
                         mc = _RE_COMPONENTS.fullmatch(typ)
                         if not mc:
-                            raise LatticeError("Invalid spin model. Unknown interaction type '%s'." % typ)
+                            raise LatticeError("unknown interaction type '%s' in spin model '%s'" % (typ, name))
                         sign = np.float32(-1.0 if mc.group(1) == "-" else 1.0)
                         st["xyz".index(mc.group(2)), "xyz".index(mc.group(3))] += sign * val
                     out.add(tuple(int(x) for x in m1.groups()), tuple(int(x) for x in m2.groups()), st)
                     if param not in out.interaction_parameters:
                         out.interaction_parameters.append(param)
                 return out
-        raise LatticeError("Invalid spin model definition.")
+        raise LatticeError("no spin model named '%s'" % name)
 
 
 def _iter_resource_roots(bundle: Optional[str]):
@@ -421,7 +421,7 @@ def _symmetry_reduce(g: _Geometry, model: SpinModelUnitCell, site1: Site, site2:
 
     nbs = g.neighbors(site2)
     if len(nbs) < 3:
-        raise LatticeError("Could not build lattice. Too few neighbors. ")
+        raise LatticeError("lattice construction: a site has fewer than three neighbours")
     n1 = nbs[0]
     n2 = None
     p2 = g.pos(site2)
@@ -430,7 +430,7 @@ def _symmetry_reduce(g: _Geometry, model: SpinModelUnitCell, site1: Site, site2:
             n2 = nbs[i]
             break
         if i == len(nbs) - 1:
-            raise LatticeError("Could not build lattice. Unfavorable lattice geometry. ")
+            raise LatticeError("lattice construction: all neighbours of a site lie on one line")
 
     pref = g.pos(ref)
     valid: List[Tuple[Mat4, Tuple[int, int, int]]] = []
@@ -473,13 +473,13 @@ def _symmetry_reduce(g: _Geometry, model: SpinModelUnitCell, site1: Site, site2:
 
                 e = _norm(_sub(_m4_apply(t, p2), pref))
                 if e > EPSILON:
-                    raise LatticeError("Lattice symmetry calculation has failed (site2 -> ref).")
+                    raise LatticeError("symmetry construction: the transform does not carry the site onto the reference site")
                 e = _norm(_cross(_normalize(_sub(_m4_apply(t, g.pos(n1)), _m4_apply(t, p2))), _normalize(_sub(pc1, pref))))
                 if e > EPSILON:
-                    raise LatticeError("Lattice symmetry calculation has failed (n1 -> c1).")
+                    raise LatticeError("symmetry construction: the transform does not align the first neighbour")
                 e = _dot(_sub(_m4_apply(t, g.pos(n2)), pref), _cross(_sub(pc1, pref), _sub(pc2, pref)))
                 if abs(e) > EPSILON:
-                    raise LatticeError("Lattice symmetry calculation has failed (n2 coplanarity).")
+                    raise LatticeError("symmetry construction: the second neighbour leaves the target plane")
 
                 if any(g.site_at(_m4_apply(t, g.pos(p))) is None for p in pre_sites):
                     continue
@@ -509,13 +509,13 @@ def _symmetry_reduce(g: _Geometry, model: SpinModelUnitCell, site1: Site, site2:
                 valid.append((t, spin_t))
 
     if not valid:
-        raise LatticeError("Lattice symmetry calculation has failed (no identity).")
+        raise LatticeError("symmetry construction: not even the identity was found valid")
     out: List[Tuple[Site, Tuple[int, int, int]]] = []
     p1 = g.pos(site1)
     for (t, perm) in valid:
         ts1 = g.site_at(_m4_apply(t, p1))
         if ts1 is None:
-            raise LatticeError("Lattice symmetry calculation has failed (invalid symmetry).")
+            raise LatticeError("symmetry construction: a valid transform maps the site outside the lattice")
         if (ts1, perm) not in out:
             out.append((ts1, perm))
     return out
@@ -640,7 +640,7 @@ def new_lattice_model(uc: LatticeUnitCell, model: SpinModelUnitCell, lattice_ran
         for n in neighborhoods[b]:
             equiv = _symmetry_reduce(g, model, n, (0, 0, 0, b), True)
             if not equiv:
-                raise LatticeError("Could not build lattice. Could not find enough symmetries")
+                raise LatticeError("lattice construction: no symmetry maps a site pair into the reduced set")
             bid, nid, eid = index[(0, 0, 0, b)], index[n], index[equiv[0][0]]
             sym_rid[bid, nid] = sym_rid[0, eid]
             p = equiv[0][1]
