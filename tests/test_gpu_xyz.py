@@ -125,3 +125,42 @@ def test_xyz_permuted_honeycomb_parity(bubble, monkeypatch):
         cut = float(p.cutoffs[k])
         _compare(p, o, cut, v2, v4, "XYZ honeycomb-kitaev-gamma r3 nw=%d L=%d O=%d (%s)"
                  % (p.nw, p.L, p.overlap_total, bubble))
+
+
+@pytest.mark.timeout(600)
+def test_xyz_honeycomb_kitaev_r8_nw64_along_flow():
+    """The XYZ core at its benchmark size (honeycomb Heisenberg-Kitaev J = 0.1,
+    K = -1, range 8: L = 23; 64 frequencies in [0.005, 50], cutoff 50 -> 0.01
+    x 0.98) on states its own GPU flow reached: every dGamma item, dSigma and
+    the four correlations at three cutoffs against the oracle's full
+    evaluation, normwise, and dGamma elementwise with a floor of 1e-3 of the
+    largest entry (whole arrays of 12 million entries: the reassociated sums
+    leave the worst entries an absolute error of a few 1e-14 of it)."""
+    import bench
+    from helpers import elem_rel
+    from oracle import XYZ_GROUPS, XyzOracle
+    from spinparser_amd.capi import XyzFlowContext
+    p = bench.build_core_problem("xyz")
+    _log("XYZ honeycomb-kitaev r8 nw64: L=%d O=%d nf=%d" % (p.L, p.overlap_total, p.n_freq))
+    o = XyzOracle(p)
+    n = len(p.cutoffs) - 1
+    ks = {0, n // 4, n // 2}
+    ctx = XyzFlowContext(p)
+    ctx.set_state(float(p.cutoffs[0]), *p.initial_state(np.float64))
+    for k in range(n // 2 + 1):
+        ctx.compute_step()
+        if k in ks:
+            cut = float(p.cutoffs[k])
+            _, v2, v4 = ctx.get_state()
+            dv2_g, dv4_g = ctx.get_flow()
+            dv2_o, dv4_o, _ = o.compute_step(cut, v2, v4, dtype=np.float64)
+            corr_g = ctx.measure_correlation()
+            corr_o = o.correlation(cut, v2, v4, dtype=np.float64)
+            e2, e4, el4 = max_rel(dv2_g, dv2_o), max_rel(dv4_g, dv4_o), elem_rel(dv4_g, dv4_o, 1e-3)
+            ec = max(max_rel(corr_g[i], corr_o[g]) for i, g in enumerate(XYZ_GROUPS) if np.abs(corr_o[g]).max() > 0)
+            _log("  k=%d cutoff %.5g: dSigma %.3g  dGamma %.3g (elementwise %.3g)  correlations %.3g"
+                 % (k, cut, e2, e4, el4, ec))
+            assert e2 < TOL and e4 < TOL and el4 < TOL and ec < TOL
+        assert not ctx.is_diverged()
+        ctx.finalize_step(float(p.cutoffs[k + 1]))
+    ctx.close()
