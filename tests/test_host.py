@@ -194,3 +194,41 @@ def test_bench_refuses_several_gpus():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_task_status_round_trip_and_corruption(tmp_path):
+    """The <calculation> element of the task file (src/TaskFileParser.cpp:33-72,
+    349-374): written and read back for every status; a status without the
+    times it requires is rejected; 'new' leaves no element behind."""
+    from spinparser_amd.frgcore import InitializationError
+    from spinparser_amd.runner import FINISHED, NEW, POSTPROCESSING, RUNNING, ComputationStatus, read_status, \
+        write_task_file
+    from spinparser_amd.task import make_task_xml
+    path = str(tmp_path / "task.xml")
+    with open(path, "w") as f:
+        f.write(make_task_xml(lattice="square", lattice_range=2, model="square-heisenberg", symmetry="SU2",
+                              couplings={"j": 1.0}, freq_min=0.1, freq_max=10.0, freq_count=4, cutoff_max=10.0,
+                              cutoff_min=1.0, cutoff_step=0.9))
+    assert read_status(path).identifier == NEW
+    for st in (RUNNING, POSTPROCESSING, FINISHED):
+        want = ComputationStatus(st, "2024-Jan-05 13:02:11", "2024-Jan-05 13:05:00",
+                                 "2024-Jan-05 13:06:00" if st == FINISHED else None)
+        write_task_file(path, want)
+        got = read_status(path)
+        assert got == want
+    write_task_file(path, ComputationStatus(NEW))
+    assert read_status(path).identifier == NEW
+    with open(path) as f:
+        assert "<calculation" not in f.read()
+    # a finished task without an end time, a running one without a start time
+    import xml.etree.ElementTree as ET
+    for attrs in ({"status": FINISHED, "startTime": "a", "checkpointTime": "b"},
+                  {"status": RUNNING}, {"status": "bogus", "startTime": "a"}):
+        tree = ET.parse(path)
+        root = tree.getroot()
+        for old in root.findall("calculation"):
+            root.remove(old)
+        ET.SubElement(root, "calculation", attrs)
+        tree.write(path)
+        with pytest.raises(InitializationError):
+            read_status(path)
