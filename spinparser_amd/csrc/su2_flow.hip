@@ -23,14 +23,15 @@
 // Kernels (one derivative evaluation = k_dsigma, k_weights, k_dgamma):
 //   k_dsigma   one wavefront per (flow, mesh point); lanes split the range sum.
 //   k_weights  one thread per (flow, exact index): window points, coefficients.
-//   k_dgamma   one 256-thread workgroup per (flow, frequency triple (s,t,u)),
+//   k_dgamma   one workgroup per (flow, frequency triple (s,t,u)): 256 threads,
+//              128 for lattices of at most 16 representative sites;
 //              blocks mapped to XCDs in contiguous item ranges.  The evaluation
 //              points of the three channels form one sequence of chunks; per chunk:
-//              (A) producer wave (wave 3), one lane per (point, pair of access
+//              (A) producer wave (the last wave), one lane per (point, pair of access
 //                  buffers): the two bracket searches a pair shares, sign /
 //                  site-exchange folding, support byte offsets and weights ->
 //                  one of two LDS descriptor buffers, one chunk ahead;
-//              (B) worker waves 0..2, (slot, rid) pairs: 4 x 4 vertex gathers
+//              (B) worker waves (the others), (slot, rid) pairs: 4 x 4 vertex gathers
 //                  (buffer loads with 32-bit offsets) and the channel algebra
 //                  from the other descriptor buffer;
 //              (C, t channel) the RPA lattice bubble accumulated as
@@ -88,6 +89,14 @@ namespace {
 #define SPFRG_MIN_BLOCKS_MIXED 4
 #endif
 constexpr int kThreads = SPFRG_THREADS;
+// small lattices: the producer's work per evaluation point does not shrink
+// with L while the workers' does, so flows with L <= kSmallL run the dGamma
+// kernel in workgroups of kSmallThreads (one worker wave per producer wave,
+// twice the workgroups per CU); measured 17-43 % faster for L = 7, 9, 11 and
+// 16, slower for L = 23 and 31
+constexpr int kSmallThreads = 128;
+constexpr int kSmallL = 16;
+inline int dgamma_threads(int L) { return L <= kSmallL ? kSmallThreads : kThreads; }
 constexpr int kMaxBatch = SPFRG_MAX_BATCH;
 constexpr int kMaxBatchMixed = SPFRG_MAX_BATCH_MIXED;
 #ifndef SPFRG_UNROLL_FP64
@@ -186,6 +195,7 @@ struct FlowDev {
                // [4] scale exponent S of the FP32 shadow (G32 = G * 2^-S), [5] largest
                // binary exponent of G written since the last plan, [6] planned S
   int nw, L, R, B, Rb, emax, nb, slots, precision, euler;
+  int nt;  // threads per workgroup of the dGamma kernel
   long long nf, item_base, nv;
   unsigned g_bytes, g32_bytes;
   double spin, cut, step;
@@ -323,8 +333,8 @@ __device__ __forceinline__ void ksum_add(KSum &k, float x) {
 // and evaluate chunk j, the producer builds the descriptors of chunk j + 1 into
 // the other LDS buffer, so the bracket searches and the t channel's site-local
 // gathers overlap the workers' gathers instead of stalling them.
-template <bool MIXED>
-__global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOCKS) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,
+template <bool MIXED, int NT>
+__global__ __launch_bounds__(NT, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOCKS) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,
                                                      long long per_xcd) {
   using V = typename std::conditional<MIXED, float, double>::type;
   using V2 = typename std::conditional<MIXED, float2, double2>::type;
@@ -368,18 +378,18 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
   while (so * (so + 1) / 2 > su) --so;
   const int uo = su - so * (so + 1) / 2;
 
-  for (int i = tid; i < nw; i += kThreads) s_mesh[i] = F.mesh[i];
-  for (int i = tid; i < L; i += kThreads) { s_site[i] = F.site_rid[i]; s_inv[i] = F.inv_rid[i]; }
-  for (int i = tid; i < 2 * Lp * Lp; i += kThreads) s_C[i] = 0.0;
-  for (int i = tid; i < 2 * nb * Lp; i += kThreads) { s_P[i] = V(0); s_Q[i] = V(0); }
-  for (int i = tid; i < 2 * L; i += kThreads) s_rpa[i] = 0.0;
+  for (int i = tid; i < nw; i += NT) s_mesh[i] = F.mesh[i];
+  for (int i = tid; i < L; i += NT) { s_site[i] = F.site_rid[i]; s_inv[i] = F.inv_rid[i]; }
+  for (int i = tid; i < 2 * Lp * Lp; i += NT) s_C[i] = 0.0;
+  for (int i = tid; i < 2 * nb * Lp; i += NT) { s_P[i] = V(0); s_Q[i] = V(0); }
+  for (int i = tid; i < 2 * L; i += NT) s_rpa[i] = 0.0;
   __syncthreads();
 
   const int lane = tid & 63;
-  const bool producer = tid >= kThreads - 64;
+  const bool producer = tid >= NT - 64;
   const int slot = tid / L;
   const int rid = tid - slot * L;
-  const bool active = slot < slots;  // slots * L <= kThreads - 64: workers only
+  const bool active = slot < slots;  // slots * L <= NT - 64: workers only
   const int sr_fwd = active ? s_site[rid] * esz : 0;
   const int sr_inv = active ? s_inv[rid] * esz : 0;
   double acc_s = 0.0, acc_d = 0.0;   // FP64 form
@@ -701,7 +711,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
       if (MIXED) {
         const int nr = Lp / 2, nc = Lp / 4;
         const int tiles = 2 * nr * nc;
-        for (int jt = tid; jt < tiles; jt += kThreads) {
+        for (int jt = tid; jt < tiles; jt += NT) {
           const int cch = jt / (nr * nc);
           const int r = jt - cch * nr * nc;
           const int a = 2 * (r / nc), b = 4 * (r - (r / nc) * nc);
@@ -736,7 +746,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
         constexpr int TR = SPFRG_TILE_R, TC = SPFRG_TILE_C;
         const int nr = Lp / TR, nc = Lp / TC;
         const int tiles = 2 * nr * nc;
-        for (int jt = tid; jt < tiles; jt += kThreads) {
+        for (int jt = tid; jt < tiles; jt += NT) {
           const int cch = jt / (nr * nc);
           const int r = jt - cch * nr * nc;
           const int a = TR * (r / nc), b = TC * (r - (r / nc) * nc);
@@ -774,7 +784,7 @@ __global__ __launch_bounds__(kThreads, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MI
       if (jc == cS + cT - 1) {
         // RPA(rid) = sum over getOverlap(rid) of C[rid1][rid2] (src/SU2/SU2FrgCore.cpp:250-266)
         const int wave = tid >> 6;
-        for (int r = wave; r < L; r += kThreads / 64) {
+        for (int r = wave; r < L; r += NT / 64) {
           double ps = 0.0, pd = 0.0;
           const int b = F.ovl_off[r], e = F.ovl_off[r + 1];
           for (int i = b + lane; i < e; i += 64) {
@@ -1179,19 +1189,26 @@ int check_problem(sp_ctx *ctx, const sp_su2_problem *p) {
   return SP_OK;
 }
 
-// worker slots (waves 0..2) and points per chunk; the producer wave holds one
-// lane per (point, pair), so nb <= 32
-// (points per slot and chunk shrink while the LDS plan misses a quarter of
-// the CU's 160 KB, so that 4 workgroups stay resident per CU)
-void launch_shape(int nw, int L, int vs, int &slots, int &nb, size_t &lds) {
-  slots = (kThreads - 64) / L;
+// LDS budget of one dGamma workgroup: a quarter of the CU's 160 KB, which
+// keeps four 256-thread workgroups (four waves per SIMD) resident.  The
+// 128-thread workgroups of small lattices keep it too: more points per chunk
+// at six or seven workgroups per CU measured 2-6 % faster than the eighth
+// of the LDS that eight resident workgroups would allow (L = 9, 11, 16)
+constexpr size_t kLdsBudget = 160 * 1024 / 4;
+
+// worker slots (all waves but the last) and points per chunk of a flow whose
+// dGamma workgroups have nt threads; the producer wave holds one lane per
+// (point, pair), so nb <= 32 (points per slot and chunk shrink while the LDS
+// plan exceeds kLdsBudget)
+void launch_shape(int nw, int L, int vs, int nt, int &slots, int &nb, size_t &lds) {
+  slots = (nt - 64) / L;
   int per = (vs == 4 ? kMaxBatchMixed : kMaxBatch) / slots;
   if (per < 1) per = 1;
   for (;;) {
     nb = slots * per;
     if (nb > 32) nb = 32;
     lds = (size_t)lds_layout(nw, L, nb, slots, vs).total;
-    if (lds <= 160 * 1024 / 4 || per == 1) break;
+    if (lds <= kLdsBudget || per == 1) break;
     --per;
   }
 }
@@ -1200,13 +1217,14 @@ size_t lds_of(const FlowDev &f) {
   return (size_t)lds_layout(f.nw, f.L, f.nb, f.slots, f.precision == SP_PRECISION_MIXED ? 4 : 8).total;
 }
 
-// Flows of one dGamma launch share its dynamic LDS size, the largest of them;
-// a flow whose plan leaves room for 4 workgroups per CU (160 KB of LDS) must
-// not run at the occupancy of a larger one.  Launch class = precision x
-// (LDS plan within that quarter or not); a batch issues one launch per class.
-constexpr size_t kLdsQuarter = 160 * 1024 / 4;  // as in launch_shape
+// Flows of one dGamma launch share its workgroup size and its dynamic LDS
+// size, the largest of them; a flow whose plan fits kLdsBudget must not run at
+// the occupancy of a larger one.  Launch class = precision x workgroup size x
+// (LDS plan within the budget or not); a batch issues one launch per class.
+constexpr int kLaunchClasses = 8;
 int launch_class(const FlowDev &f) {
-  return (f.precision == SP_PRECISION_MIXED ? 2 : 0) + (lds_of(f) > kLdsQuarter ? 1 : 0);
+  return (f.precision == SP_PRECISION_MIXED ? 2 : 0) + (lds_of(f) > kLdsBudget ? 1 : 0) +
+         (f.nt == kSmallThreads ? 4 : 0);
 }
 
 // Launch geometry of a set of flows held in a host array (h) whose device copy is d.
@@ -1256,10 +1274,19 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
       const long long per_xcd = 0;  // unused by the chunked mapping
       const long long rounds = (total + 8 * kChunk - 1) / (8 * kChunk);
       const unsigned grid = (unsigned)(rounds * 8 * kChunk);
-      if (mixed)
-        hipLaunchKernelGGL(k_dgamma<true>, dim3(grid), dim3(kThreads), lds, st, d + i0, i1 - i0, total, per_xcd);
+      const int nt = h[i0].nt;
+      if (mixed && nt == kSmallThreads)
+        hipLaunchKernelGGL((k_dgamma<true, kSmallThreads>), dim3(grid), dim3(nt), lds, st, d + i0, i1 - i0, total,
+                           per_xcd);
+      else if (mixed)
+        hipLaunchKernelGGL((k_dgamma<true, kThreads>), dim3(grid), dim3(nt), lds, st, d + i0, i1 - i0, total,
+                           per_xcd);
+      else if (nt == kSmallThreads)
+        hipLaunchKernelGGL((k_dgamma<false, kSmallThreads>), dim3(grid), dim3(nt), lds, st, d + i0, i1 - i0, total,
+                           per_xcd);
       else
-        hipLaunchKernelGGL(k_dgamma<false>, dim3(grid), dim3(kThreads), lds, st, d + i0, i1 - i0, total, per_xcd);
+        hipLaunchKernelGGL((k_dgamma<false, kThreads>), dim3(grid), dim3(nt), lds, st, d + i0, i1 - i0, total,
+                           per_xcd);
       e = hipGetLastError();
       if (e != hipSuccess) return e;
     }
@@ -1355,7 +1382,8 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   f.spin = p->spin;
   f.precision = p->precision;
   f.item_base = 0;
-  launch_shape(f.nw, f.L, f.precision == SP_PRECISION_MIXED ? 4 : 8, f.slots, f.nb, ctx->lds);
+  f.nt = dgamma_threads(f.L);
+  launch_shape(f.nw, f.L, f.precision == SP_PRECISION_MIXED ? 4 : 8, f.nt, f.slots, f.nb, ctx->lds);
   int max_lds = 0;
   SP_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
   if (ctx->lds > (size_t)max_lds) return fail(ctx, SP_EARG, "problem too large for the LDS plan of the dGamma kernel");
@@ -1363,7 +1391,7 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
     // a mixed context may switch to the FP64 form (sp_set_precision)
     int s64, nb64;
     size_t lds64;
-    launch_shape(f.nw, f.L, 8, s64, nb64, lds64);
+    launch_shape(f.nw, f.L, 8, f.nt, s64, nb64, lds64);
     if (lds64 > (size_t)max_lds) return fail(ctx, SP_EARG, "problem too large for the LDS plan of the dGamma kernel");
   }
   if (f.nv * 16 >= (1LL << 31)) return fail(ctx, SP_EARG, "vertex too large for 32-bit byte offsets");
@@ -1439,8 +1467,14 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   SP_HIP(hipMemset(f.flags, 0, 32));
   SP_HIP(hipMemset(f.Wn, 0, f.nw * 4));
   SP_HIP(hipMemset(f.Cn, 0, 4));
-  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false, kThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true, kThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false, kSmallThreads>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true, kSmallThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             max_lds));
   SP_HIP(hipMemcpy(ctx->d_fd, &f, sizeof(FlowDev), hipMemcpyHostToDevice));
   return SP_OK;
 }
@@ -1493,7 +1527,7 @@ int sp_set_precision(sp_ctx *ctx, int precision) {
     return fail(ctx, SP_EARG, "the context holds no FP32 shadow (create it with SP_PRECISION_MIXED)");
   if (precision == f.precision) return SP_OK;
   f.precision = precision;
-  launch_shape(f.nw, f.L, precision == SP_PRECISION_MIXED ? 4 : 8, f.slots, f.nb, ctx->lds);
+  launch_shape(f.nw, f.L, precision == SP_PRECISION_MIXED ? 4 : 8, f.nt, f.slots, f.nb, ctx->lds);
   int rc = upload_fd(ctx);
   if (rc) return rc;
   return rebuild_shadow(ctx);
@@ -1735,7 +1769,7 @@ static int group_prepare(sp_group *grp, sp_ctx *const *ctxs, int n, size_t *out_
   if (n < 1 || n > grp->capacity || !ctxs) return gfail(grp, SP_EARG, "invalid flow count");
   // FP64 flows first, then mixed; item bases restart per precision class
   std::vector<int> order;
-  for (int cls = 0; cls < 4; ++cls)
+  for (int cls = 0; cls < kLaunchClasses; ++cls)
     for (int i = 0; i < n; ++i)
       if (ctxs[i] && launch_class(ctxs[i]->fd) == cls) order.push_back(i);
   if ((int)order.size() != n) return gfail(grp, SP_EARG, "null context in group");
@@ -1909,8 +1943,10 @@ int noalloc_scratch(int dev, unsigned char **base, int *max_lds) {
     if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return SP_EHIP;
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return SP_EHIP;
-    if (hipFuncSetAttribute((const void *)k_dgamma<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) !=
-        hipSuccess)
+    if (hipFuncSetAttribute((const void *)k_dgamma<false, kThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lds) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_dgamma<false, kSmallThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lds) != hipSuccess)
       return SP_EHIP;
     g_noalloc_lds[dev] = lds;
     g_noalloc_ptr[dev] = static_cast<unsigned char *>(q);
@@ -1945,7 +1981,8 @@ extern "C" int sp_compute_step_noalloc(const sp_su2_problem *p, double cutoff, c
   f.g_bytes = (unsigned)(f.nv * 16);
   f.cut = cutoff;
   size_t lds = 0;
-  launch_shape(f.nw, f.L, 8, f.slots, f.nb, lds);
+  f.nt = dgamma_threads(f.L);
+  launch_shape(f.nw, f.L, 8, f.nt, f.slots, f.nb, lds);
   double mesh_d[256];
   for (int i = 0; i < f.nw; ++i) mesh_d[i] = (double)p->mesh[i];
   geo_mesh_params(mesh_d, f.nw, f.geo, f.ga, f.gb);
