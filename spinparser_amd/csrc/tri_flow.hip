@@ -21,8 +21,9 @@
 // density-spin components carrying a factor i); no coefficient is spelled out
 // by hand.  Each table expands into straight-line FMAs on register arrays.
 //
-// dGamma kernel: one workgroup of 256 threads per frequency triple (s, t, u),
-// items walked t-major in XCD runs.  Per channel the evaluation points are
+// dGamma kernel: one workgroup per frequency triple (s, t, u), 128 threads for
+// lattices of at most 32 sites and 256 above, items walked t-major in XCD
+// runs.  Per channel the evaluation points are
 // processed in chunks of nb points:
 //   A  one thread per (point, access buffer) builds the descriptor (4 supports,
 //      weights, exchange / sign flags), the T channel's four site-local
@@ -56,6 +57,13 @@ namespace {
 using namespace frg;
 
 constexpr int kTriThreads = 256;
+// dGamma workgroups of 128 threads (three evaluation points per chunk at
+// L = 19) for lattices of at most kTriSmallL sites: four resident workgroups
+// per CU instead of two overlap one another's barrier-separated phases
+// (kagome-DM r5: 100.0 against 106.8 ms at the smallest cutoff, 3.6 against
+// 5.8 ms at the largest; 64 and 192 threads were slower)
+constexpr int kTriSmallThreads = 128;
+constexpr int kTriSmallL = 32;
 constexpr int kTriMaxPoints = 16;  // evaluation points per chunk
 #ifndef SPFRG_TRI_CHUNK
 #define SPFRG_TRI_CHUNK 16
@@ -216,6 +224,7 @@ struct TriDev {
   double *corr_partial, *corr_out;
   int *flags;  // [0] NaN in the flow, [1] evaluation-list overflow
   int nw, L, R, B, Rb, emax, nb, euler;
+  int nt;  // threads per workgroup of the dGamma kernel
   long long nf, nv;
   unsigned g_bytes;
   double cut, step;
@@ -372,7 +381,8 @@ __device__ __forceinline__ int tri_support(int so, int to, int uo, int nw, int F
   return row * FS;
 }
 
-__global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev F) {
+template <int NT>
+__global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev F) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const long long total = F.nf;
   // workgroups are dispatched round-robin over the 8 XCDs: each XCD receives
@@ -411,11 +421,11 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
   while (so * (so + 1) / 2 > su) --so;
   const int uo = su - so * (so + 1) / 2;
 
-  for (int i = tid; i < nw; i += kTriThreads) s_mesh[i] = F.mesh[i];
+  for (int i = tid; i < nw; i += NT) s_mesh[i] = F.mesh[i];
   // stored component offsets (component, site) for the plain and the exchanged
   // bundle: components swapped on exchange, spin indices through the site's
   // permutation (TRIVertexTwoParticle.hpp:368-388)
-  for (int q = tid; q < 16 * L; q += kTriThreads) {
+  for (int q = tid; q < 16 * L; q += NT) {
     const int c = q / L, j = q - (q / L) * L;
     const int s1 = c / 4, s2 = c % 4;
     const int *pf = F.site_perm + 3 * j, *pi = F.inv_perm + 3 * j;
@@ -491,7 +501,7 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
       // channel also the site-local vertices, one thread per (point, local, component)
       const int nbq = nvalid * 4 * L;
       const int nq = nbq + (ch == 1 ? nvalid * 4 * 16 : 0);
-      for (int q = tid; q < nq; q += kTriThreads) {
+      for (int q = tid; q < nq; q += NT) {
         if (q >= nbq) {
           // getValueLocal (TRIVertexTwoParticle.hpp:300-309): site 0; the sign
           // table is indexed with the exchanged components
@@ -634,7 +644,7 @@ __global__ __launch_bounds__(kTriThreads, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamm
   }
   __syncthreads();
   const int ng = nb * 2;
-  for (int q = tid; q < 16 * L; q += kTriThreads) {
+  for (int q = tid; q < 16 * L; q += NT) {
     double v = 0.0;
     for (int g = 0; g < ng; ++g) v += red[(size_t)g * 16 * L + q];
     v *= (1.0 / kTwoPi);
@@ -929,8 +939,11 @@ hipError_t tri_launch_compute(sp_tri_ctx *ctx) {
   if (e != hipSuccess) return e;
   TriProf ps(ctx, 0);
   const long long rounds = (F.nf + 8 * kTriChunk - 1) / (8 * kTriChunk);
-  hipLaunchKernelGGL(k_tri_dgamma, dim3((unsigned)(rounds * 8 * kTriChunk)), dim3(kTriThreads), ctx->lds, ctx->stream,
-                     F);
+  const dim3 grid((unsigned)(rounds * 8 * kTriChunk));
+  if (F.nt == kTriSmallThreads)
+    hipLaunchKernelGGL(k_tri_dgamma<kTriSmallThreads>, grid, dim3(kTriSmallThreads), ctx->lds, ctx->stream, F);
+  else
+    hipLaunchKernelGGL(k_tri_dgamma<kTriThreads>, grid, dim3(kTriThreads), ctx->lds, ctx->stream, F);
   return hipGetLastError();
 }
 
@@ -971,7 +984,8 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   f.nf = (long long)p->nw * p->nw * (p->nw + 1) / 2;
   f.nv = f.nf * 16 * p->L;
   f.emax = 2 * p->nw + 8;
-  int nb = kTriThreads / (2 * p->L);
+  f.nt = p->L <= kTriSmallL ? kTriSmallThreads : kTriThreads;
+  int nb = f.nt / (2 * p->L);
   if (nb > kTriMaxPoints) nb = kTriMaxPoints;
   if (nb < 1) nb = 1;
   f.nb = nb;
@@ -1065,7 +1079,10 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   SP_THIP(hipMemset(f.flags, 0, 16));
   SP_THIP(hipMemset(f.Wn, 0, f.nw * 4));
   SP_THIP(hipMemset(f.Cn, 0, 4));
-  SP_THIP(hipFuncSetAttribute((const void *)k_tri_dgamma, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_THIP(hipFuncSetAttribute((const void *)k_tri_dgamma<kTriThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              max_lds));
+  SP_THIP(hipFuncSetAttribute((const void *)k_tri_dgamma<kTriSmallThreads>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   return SP_OK;
 }
 
