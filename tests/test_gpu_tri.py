@@ -175,3 +175,38 @@ def test_tri_dm_flow_spin_symmetries():
               "TRICorYY": 1, "TRICorYZ": 1, "TRICorZX": -1, "TRICorZY": 1, "TRICorZZ": 1}
     for name, sgn in parity.items():
         assert np.abs(a[:, g[name]] - sgn * b[:, g[name]]).max() < tol, name
+
+
+def test_tri_large_lattice_workgroup_shape():
+    """Kagome-DM range 7 (L = 34): above 32 sites the dGamma kernel runs in
+    256-thread workgroups (six points per chunk at most) instead of 128; every
+    dGamma item, dSigma and the ten correlations on the GPU flow's own states at
+    three cutoffs against the oracle's full evaluation."""
+    from helpers import elem_rel, tri_problem_of
+    from oracle import TRI_GROUPS, TriOracle
+    from spinparser_amd.capi import TriFlowContext
+    p = tri_problem_of("kagome", 7, "kagome-DM", {"j": 1.0, "d": 0.2}, 8, 0.005, 50.0, 50.0, 0.01, 0.98)
+    assert p.L > 32
+    _log("TRI kagome-DM r7 nw8: L=%d O=%d" % (p.L, p.overlap_total))
+    o = TriOracle(p)
+    n = len(p.cutoffs) - 1
+    ks = {0, n // 2, n - 1}
+    ctx = TriFlowContext(p)
+    ctx.set_state(float(p.cutoffs[0]), *p.initial_state(np.float64))
+    for k in range(n):
+        ctx.compute_step()
+        if k in ks:
+            cut = float(p.cutoffs[k])
+            _, v2, v4 = ctx.get_state()
+            dv2, dv4 = ctx.get_flow()
+            r2, r4, _ = o.compute_step(cut, v2, v4)
+            corr_g = ctx.measure_correlation()
+            corr_o = o.correlation(cut, v2, v4, dtype=np.float64)
+            e2, e4, el4 = max_rel(dv2, r2), max_rel(dv4, r4), elem_rel(dv4, r4, 1e-3)
+            ec = max(max_rel(corr_g[i], corr_o[g]) for i, g in enumerate(TRI_GROUPS) if np.abs(corr_o[g]).max() > 0)
+            _log("  k=%d cutoff %.4g: dSigma %.3g  dGamma %.3g (elementwise %.3g)  correlations %.3g"
+                 % (k, cut, e2, e4, el4, ec))
+            assert e2 < TOL and e4 < TOL and el4 < TOL and ec < TOL
+        assert not ctx.is_diverged()
+        ctx.finalize_step(float(p.cutoffs[k + 1]))
+    ctx.close()
