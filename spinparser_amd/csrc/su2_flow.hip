@@ -327,8 +327,9 @@ __device__ __forceinline__ void ksum_add(KSum &k, float x) {
   k.s = t;
 }
 
-// dGamma kernel.  Waves 0..2 are workers, (slot, rid) pairs with slots =
-// floor(192 / L); wave 3 is the producer.  The evaluation points of the three
+// dGamma kernel in workgroups of NT threads.  All waves but the last are
+// workers, (slot, rid) pairs with slots = floor((NT - 64) / L); the last wave
+// is the producer.  The evaluation points of the three
 // channels form one sequence of chunks of nb points; while the workers gather
 // and evaluate chunk j, the producer builds the descriptors of chunk j + 1 into
 // the other LDS buffer, so the bracket searches and the t channel's site-local
