@@ -225,6 +225,7 @@ struct TriDev {
   int *flags;  // [0] NaN in the flow, [1] evaluation-list overflow
   int nw, L, R, B, Rb, emax, nb, euler;
   int nt;  // threads per workgroup of the dGamma kernel
+  int cs;  // component stride of its LDS bundles compiled into it (doubles); 0: L, read at run time
   long long nf, nv;
   unsigned g_bytes;
   double cut, step;
@@ -232,38 +233,64 @@ struct TriDev {
   float ga, gb;
 };
 
-// LDS stride of one 16 x L bundle (doubles).  The padding keeps bundle bases
-// off a common bank: the lattice bubble's lanes read the same site of
-// different bundles at once, and a stride that is a multiple of 16 doubles
-// puts every bundle on the same banks (measured: 1 % faster at L = 19)
 // lattice bubble: load the next overlap entry's packed sites ahead (1) or
 // right before use (0)
 #ifndef SPFRG_TRI_PREFETCH
 #define SPFRG_TRI_PREFETCH 1
 #endif
+// padding of the run-time bundle stride: bundle bases off a common bank (the
+// lattice bubble's lanes read the same site of different bundles at once; a
+// stride that is a multiple of 16 doubles puts every bundle on the same
+// banks; measured 1 % faster at L = 19)
 #ifndef SPFRG_TRI_BPAD
 #define SPFRG_TRI_BPAD 6
 #endif
-__host__ __device__ inline int tri_bundle_stride(int L) { return 16 * L + SPFRG_TRI_BPAD; }
+// padding of the compiled bundle stride (doubles)
+#ifndef SPFRG_TRI_CPAD
+#define SPFRG_TRI_CPAD 2
+#endif
+
+// Geometry of the 16-component bundles in LDS: component (s1, s2) of site j
+// of bundle d sits at d * bs + s1 * rs + s2 * ks + j (doubles).
+//  * cs = 0 (site count L read at run time): rs = 4 L, ks = L, bundles of
+//    16 L + pad doubles one after the other.
+//  * cs > 0 (site stride cs >= L and nb compiled into the kernel): rs = cs,
+//    bs = 4 cs + pad and the second spin index outermost, ks = 4 nb bs, made
+//    odd.  All component offsets are then instruction immediates, and the
+//    four components of one row s1 lie more than 255 doubles apart (4 nb bs >
+//    255 for every compiled shape): the compiler cannot pair them into
+//    ds_read2_b64, whose random-address reads cost the LDS about twice the
+//    cycles of two single 8-byte reads on gfx950 (measured with the pairs:
+//    SQ_LDS_IDX_ACTIVE +83 %, 8 % slower kernel).  ks is odd, so no two of the
+//    offsets are multiples of 64 doubles either (no ds_read2st64_b64).
+struct TriBundle {
+  int rs, ks, bs, size;  // size: doubles of the whole bundle area
+};
+__host__ __device__ constexpr TriBundle tri_bundle(int L, int cs, int nb) {
+  if (cs == 0) return TriBundle{4 * L, L, 16 * L + SPFRG_TRI_BPAD, nb * 4 * (16 * L + SPFRG_TRI_BPAD)};
+  const int bs = 4 * cs + SPFRG_TRI_CPAD, ks = (nb * 4 * bs) | 1;
+  return TriBundle{cs, ks, bs, 4 * ks};
+}
 
 // LDS carve-up of k_tri_dgamma (bytes)
 struct TriLds {
   int mesh, cf, ci, off, w, fl, c, loc, A, total;
 };
-__host__ __device__ inline TriLds tri_lds(int nw, int L, int nb) {
+__host__ __device__ inline TriLds tri_lds(int nw, int L, int cs, int nb) {
   TriLds l;
   int o = 0;
   auto take = [&](int bytes) { int r = o; o += (bytes + 15) & ~15; return r; };
+  const int ss = cs > 0 ? cs : L;  // site stride of the component-offset tables
   l.mesh = take(nw * 8);
-  l.cf = take(16 * L * 4);
-  l.ci = take(16 * L * 4);
+  l.cf = take(16 * ss * 4);
+  l.ci = take(16 * ss * 4);
   l.off = take(nb * 8 * 4 * 4);
   l.w = take(nb * 8 * 4 * 8);
   l.fl = take(nb * 8 * 4);
   l.c = take(nb * 8);
   l.loc = take(nb * 4 * 16 * 8);
-  const int bs = tri_bundle_stride(L), red = nb * 2 * 16 * L;
-  l.A = take((nb * 4 * bs > red ? nb * 4 * bs : red) * 8);  // also holds the final reduction
+  const int bun = tri_bundle(L, cs, nb).size, red = nb * 2 * 16 * L;
+  l.A = take((bun > red ? bun : red) * 8);  // also holds the final reduction
   l.total = o;
   return l;
 }
@@ -381,7 +408,9 @@ __device__ __forceinline__ int tri_support(int so, int to, int uo, int nw, int F
   return row * FS;
 }
 
-template <int NT>
+// CS, NB > 0: site stride of the bundles and evaluation points per chunk
+// compiled in (F.nb == NB); CS = NB = 0: both read at run time
+template <int NT, int CS, int NB>
 __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev F) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const long long total = F.nf;
@@ -394,12 +423,20 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
   const long long pos = ((j0 / kTriChunk) * 8 + (blockIdx.x & 7)) * kTriChunk + (j0 % kTriChunk);
   if (pos >= total) return;
 
-  const int nw = F.nw, L = F.L, nb = F.nb;
+  const int nw = F.nw, L = F.L, nb = NB > 0 ? NB : F.nb;
   const long long nsu = total / nw;
   const long long item = (pos % nsu) * nw + pos / nsu;
   const int FS = 16 * L;
-  const int BS = tri_bundle_stride(L);
-  const TriLds lay = tri_lds(nw, L, nb);
+  // bundle geometry: compile-time constants for CS > 0, so that the
+  // component offsets of every bundle read and write are immediates (the
+  // lattice bubble's 32 reads per overlap entry then need 6 address
+  // additions instead of 38)
+  const TriBundle geo = tri_bundle(L, CS, nb);
+  const int RS = geo.rs, KS = geo.ks, BS = geo.bs;
+  const int SS = CS > 0 ? CS : L;  // site stride of s_cf, s_ci
+  // offset of component k = 4 s1 + s2 within a bundle
+  auto comp = [&](int k) { return CS > 0 ? (k >> 2) * RS + (k & 3) * KS : k * L; };
+  const TriLds lay = tri_lds(nw, L, CS, nb);
   double *s_mesh = reinterpret_cast<double *>(lds_raw + lay.mesh);
   int *s_cf = reinterpret_cast<int *>(lds_raw + lay.cf);
   int *s_ci = reinterpret_cast<int *>(lds_raw + lay.ci);
@@ -430,9 +467,9 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
     const int s1 = c / 4, s2 = c % 4;
     const int *pf = F.site_perm + 3 * j, *pi = F.inv_perm + 3 * j;
     const int a = s1 < 3 ? pf[s1] : 3, b = s2 < 3 ? pf[s2] : 3;
-    s_cf[q] = (4 * a + b) * L + F.site_rid[j];
+    s_cf[c * SS + j] = (4 * a + b) * L + F.site_rid[j];
     const int ai = s2 < 3 ? pi[s2] : 3, bi = s1 < 3 ? pi[s1] : 3;
-    s_ci[q] = (4 * ai + bi) * L + F.inv_rid[j];
+    s_ci[c * SS + j] = (4 * ai + bi) * L + F.inv_rid[j];
   }
   __syncthreads();
 
@@ -547,14 +584,14 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
               const int cc = 4 * s1 + s2;
               const double tsign = tu ? double(zeta(s1) * zeta(s2)) : 1.0;
               const double fsign = -double(zeta(exch ? s1 : s2));
-              const int co = cofs[cc * L + j];
+              const int co = cofs[cc * SS + j];
               double v = 0.0;
 #pragma unroll
               for (int r = 0; r < 4; ++r) {
                 const double sg = ((fl >> (2 + r)) & 1) ? tsign * fsign : tsign;
                 v = fma(sg * w[r], gload(rG, (off[r] + co) * 8), v);
               }
-              dst[cc * L] = v;
+              dst[comp(cc)] = v;
             }
           }
         }
@@ -568,7 +605,7 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
         double x[16], y[16];
         if (ch != 1) {
 #pragma unroll
-          for (int k = 0; k < 16; ++k) { x[k] = cw * X[k * L + cj]; y[k] = Y[k * L + cj]; }
+          for (int k = 0; k < 16; ++k) { x[k] = cw * X[comp(k) + cj]; y[k] = Y[comp(k) + cj]; }
           if (ch == 0) apply_table<0>(acc, x, y);
           else apply_table<1>(acc, x, y);
         } else {
@@ -588,8 +625,8 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
             const int4 gr = F.bgrp[gi];
             const int code = gr.z;
             // component rows (4 components, stride L) of A and B
-            const int xr[4] = {4 * (code & 3) * L, 4 * ((code >> 2) & 3) * L, 4 * ((code >> 4) & 3) * L, 12 * L};
-            const int yr[4] = {4 * ((code >> 6) & 3) * L, 4 * ((code >> 8) & 3) * L, 4 * ((code >> 10) & 3) * L, 12 * L};
+            const int xr[4] = {(code & 3) * RS, ((code >> 2) & 3) * RS, ((code >> 4) & 3) * RS, 3 * RS};
+            const int yr[4] = {((code >> 6) & 3) * RS, ((code >> 8) & 3) * RS, ((code >> 10) & 3) * RS, 3 * RS};
             double o[16];
 #pragma unroll
             for (int k = 0; k < 16; ++k) o[k] = 0.0;
@@ -602,19 +639,36 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
               if (SPFRG_TRI_PREFETCH && i + 1 < gr.y) pk_next = F.bent[i + 1];
               const double *xa = X + (pk & 0xffff);
               const double *ya = Y + (pk >> 16);
-              double bv[16];
+              // all 32 operands are loaded first (the scheduling barrier keeps
+              // the loads ahead of the products): with compiled offsets the
+              // compiler would otherwise reuse the registers of A's rows and
+              // wait for each row in turn (2 % slower at L = 19); with run-time
+              // offsets the interleaved form keeps the address registers down
+              double bv[16], av[16];
 #pragma unroll
               for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-                for (int cc = 0; cc < 4; ++cc) bv[4 * mm + cc] = ya[yr[mm] + cc * L];
+                for (int cc = 0; cc < 4; ++cc) bv[4 * mm + cc] = ya[yr[mm] + cc * KS];
+              if constexpr (CS > 0) {
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+                  for (int mm = 0; mm < 4; ++mm) av[4 * rr + mm] = xa[xr[rr] + mm * KS];
+                __builtin_amdgcn_sched_barrier(0);
+              }
 #pragma unroll
               for (int rr = 0; rr < 4; ++rr) {
-                const double av[4] = {xa[xr[rr]], xa[xr[rr] + L], xa[xr[rr] + 2 * L], xa[xr[rr] + 3 * L]};
+                if constexpr (CS == 0) {
+#pragma unroll
+                  for (int mm = 0; mm < 4; ++mm) av[4 * rr + mm] = xa[xr[rr] + mm * KS];
+                }
 #pragma unroll
                 for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-                  for (int cc = 0; cc < 4; ++cc)
-                    o[4 * rr + cc] = fma(bubble_sign(rr, mm, cc) > 0 ? av[mm] : -av[mm], bv[4 * mm + cc], o[4 * rr + cc]);
+                  for (int cc = 0; cc < 4; ++cc) {
+                    const double a = av[4 * rr + mm];
+                    o[4 * rr + cc] = fma(bubble_sign(rr, mm, cc) > 0 ? a : -a, bv[4 * mm + cc], o[4 * rr + cc]);
+                  }
               }
             }
             const int p0 = (code >> 12) & 3, p1 = (code >> 14) & 3, p2 = (code >> 16) & 3;
@@ -630,12 +684,12 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
           // chalice (pair 0: bundle 0 with local 4; pair 1: bundle 2 with local 5)
           const double *lc = s_loc + (ce * 4 + cpair) * 16;
 #pragma unroll
-          for (int k = 0; k < 16; ++k) { x[k] = cw * X[k * L + cj]; y[k] = lc[k]; }
+          for (int k = 0; k < 16; ++k) { x[k] = cw * X[comp(k) + cj]; y[k] = lc[k]; }
           apply_table<2>(acc, x, y);
           // inverse chalice (pair 0: local 6 with bundle 1; pair 1: local 7 with bundle 3)
           const double *li = s_loc + (ce * 4 + 2 + cpair) * 16;
 #pragma unroll
-          for (int k = 0; k < 16; ++k) { x[k] = cw * li[k]; y[k] = Y[k * L + cj]; }
+          for (int k = 0; k < 16; ++k) { x[k] = cw * li[k]; y[k] = Y[comp(k) + cj]; }
           apply_table<3>(acc, x, y);
         }
       }
@@ -931,6 +985,38 @@ int check_tri_problem(sp_tri_ctx *ctx, const sp_tri_problem *p) {
   return SP_OK;
 }
 
+// shapes (site stride, points per chunk) compiled into the 128-thread dGamma
+// kernel, packed as cs * 32 + nb: L = 13..32 rounded up to a multiple of 4
+// with nb = 64 / L (L = 21, (24, 3), exceeds the LDS budget).  A shape is used
+// while its workgroup stays within kTriLdsBudget (four resident per CU);
+// other lattices, and the 256-thread kernel, read L and nb at run time
+constexpr int kTriLdsBudget = 160 * 1024 / 4;
+constexpr int tri_shape(int cs, int nb) { return cs * 32 + nb; }
+using TriShapes = std::integer_sequence<int, 0, tri_shape(16, 4), tri_shape(20, 3), tri_shape(24, 2), tri_shape(28, 2),
+                                        tri_shape(32, 2)>;
+static_assert(kTriSmallL == 32, "TriShapes covers the lattices up to kTriSmallL sites");
+
+template <int... SH> constexpr bool tri_shape_compiled(int cs, int nb, std::integer_sequence<int, SH...>) {
+  return ((SH != 0 && SH == tri_shape(cs, nb)) || ...);
+}
+template <int SH> bool launch_tri_small(const TriDev &F, dim3 grid, size_t lds, hipStream_t st) {
+  if (SH != (F.cs ? tri_shape(F.cs, F.nb) : 0)) return false;
+  hipLaunchKernelGGL((k_tri_dgamma<kTriSmallThreads, SH / 32, SH % 32>), grid, dim3(kTriSmallThreads), lds, st, F);
+  return true;
+}
+template <int... SH>
+void launch_tri_small(const TriDev &F, dim3 grid, size_t lds, hipStream_t st, std::integer_sequence<int, SH...>) {
+  (launch_tri_small<SH>(F, grid, lds, st) || ...);
+}
+template <int... SH> hipError_t tri_allow_lds(int bytes, std::integer_sequence<int, SH...>) {
+  hipError_t e = hipSuccess;
+  ((e == hipSuccess ? (e = hipFuncSetAttribute((const void *)k_tri_dgamma<kTriSmallThreads, SH / 32, SH % 32>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes))
+                    : e),
+   ...);
+  return e;
+}
+
 hipError_t tri_launch_compute(sp_tri_ctx *ctx) {
   TriDev &F = ctx->fd;
   F.cut = ctx->cutoff;
@@ -948,9 +1034,9 @@ hipError_t tri_launch_compute(sp_tri_ctx *ctx) {
   const long long rounds = (F.nf + 8 * kTriChunk - 1) / (8 * kTriChunk);
   const dim3 grid((unsigned)(rounds * 8 * kTriChunk));
   if (F.nt == kTriSmallThreads)
-    hipLaunchKernelGGL(k_tri_dgamma<kTriSmallThreads>, grid, dim3(kTriSmallThreads), ctx->lds, ctx->stream, F);
+    launch_tri_small(F, grid, ctx->lds, ctx->stream, TriShapes{});
   else
-    hipLaunchKernelGGL(k_tri_dgamma<kTriThreads>, grid, dim3(kTriThreads), ctx->lds, ctx->stream, F);
+    hipLaunchKernelGGL((k_tri_dgamma<kTriThreads, 0, 0>), grid, dim3(kTriThreads), ctx->lds, ctx->stream, F);
   return hipGetLastError();
 }
 
@@ -996,7 +1082,12 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   if (nb > kTriMaxPoints) nb = kTriMaxPoints;
   if (nb < 1) nb = 1;
   f.nb = nb;
-  ctx->lds = (size_t)tri_lds(f.nw, f.L, f.nb).total;
+  f.cs = 0;
+  if (f.nt == kTriSmallThreads) {
+    const int cs = (f.L + 3) / 4 * 4;
+    if (tri_shape_compiled(cs, f.nb, TriShapes{}) && tri_lds(f.nw, f.L, cs, f.nb).total <= kTriLdsBudget) f.cs = cs;
+  }
+  ctx->lds = (size_t)tri_lds(f.nw, f.L, f.cs, f.nb).total;
   int max_lds = 0;
   SP_THIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
   if (ctx->lds > (size_t)max_lds) return tfail(ctx, SP_EARG, "problem too large for the LDS plan of the dGamma kernel");
@@ -1086,10 +1177,9 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   SP_THIP(hipMemset(f.flags, 0, 16));
   SP_THIP(hipMemset(f.Wn, 0, f.nw * 4));
   SP_THIP(hipMemset(f.Cn, 0, 4));
-  SP_THIP(hipFuncSetAttribute((const void *)k_tri_dgamma<kTriThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              max_lds));
-  SP_THIP(hipFuncSetAttribute((const void *)k_tri_dgamma<kTriSmallThreads>,
+  SP_THIP(hipFuncSetAttribute((const void *)k_tri_dgamma<kTriThreads, 0, 0>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_THIP(tri_allow_lds(max_lds, TriShapes{}));
   return SP_OK;
 }
 

@@ -210,3 +210,47 @@ def test_tri_large_lattice_workgroup_shape():
         assert not ctx.is_diverged()
         ctx.finalize_step(float(p.cutoffs[k + 1]))
     ctx.close()
+
+
+# one lattice per workgroup shape of the 128-thread dGamma kernel: compiled
+# (site stride, points per chunk) = (L rounded up to 4, 64 // L) for L = 13..32
+# (L = 21 excepted), and the run-time form below 13 sites
+SHAPE_CASES = [
+    ("kagome", 3, "kagome-DM", {"j": 1.0, "d": 0.2}, 8, "run-time L and nb"),
+    ("kagome", 4, "kagome-DM", {"j": 1.0, "d": 0.2}, 13, "stride 16, 4 points"),
+    ("triangular", 7, "triangular-heisenberg", {"j": 1.0}, 20, "stride 20, 3 points"),
+    ("cubic", 6, "cubic-heisenberg", {"j": 1.0}, 23, "stride 24, 2 points"),
+    ("kagome", 6, "kagome-DM", {"j": 1.0, "d": 0.2}, 26, "stride 28, 2 points"),
+    ("cubic", 7, "cubic-heisenberg", {"j": 1.0}, 31, "stride 32, 2 points"),
+]
+
+
+@pytest.mark.parametrize("case", SHAPE_CASES, ids=[c[5].replace(" ", "-").replace(",", "") for c in SHAPE_CASES])
+def test_tri_compiled_shapes(case):
+    """Every dGamma item and dSigma on a 4-point mesh against the oracle, at the
+    initial state and at a frequency-dependent state (one oracle Euler step of
+    it), for each bundle geometry of the 128-thread kernel."""
+    from helpers import elem_rel, tri_problem_of
+    from oracle import TriOracle
+    from spinparser_amd.capi import TriFlowContext
+    lattice, rng, model, cpl, L, shape = case
+    p = tri_problem_of(lattice, rng, model, cpl, 4, 0.01, 20.0, 20.0, 0.05, 0.8)
+    assert p.L == L
+    o = TriOracle(p)
+    ctx = TriFlowContext(p)
+    v2, v4 = p.initial_state(np.float64)
+    for k in (0, len(p.cutoffs) // 2):
+        cut = float(p.cutoffs[k])
+        if k > 0:
+            dv2, dv4, _ = o.compute_step(cut, v2, v4, dtype=np.float64, nthreads=8)
+            v2, v4 = v2 - 0.05 * cut * dv2, v4 - 0.05 * cut * dv4
+        _log("TRI %s r%d L=%d O=%d (%s) at cutoff %.4g" % (lattice, rng, p.L, p.overlap_total, shape, cut))
+        r2, r4, _ = o.compute_step(cut, v2, v4, dtype=np.float64, nthreads=8)
+        ctx.set_state(cut, v2, v4)
+        ctx.compute_step()
+        g2, g4 = ctx.get_flow()
+        e2, e4, el4 = max_rel(g2, r2), max_rel(g4, r4), elem_rel(g4, r4, 1e-3)
+        _log("  dSigma %.3g  dGamma %.3g (elementwise %.3g)" % (e2, e4, el4))
+        assert np.abs(r4).max() > 0
+        assert e2 < TOL and e4 < TOL and el4 < TOL
+    ctx.close()
