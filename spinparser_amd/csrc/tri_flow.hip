@@ -216,6 +216,9 @@ struct TriDev {
   // bent = rid1 | rid2 << 16 per entry
   const int *bgrp_off, *bent;
   const int4 *bgrp;
+  // sites in decreasing order of their overlap-entry counts (channel-algebra
+  // lanes take them in this order)
+  const int *site_order;
   const int *range_fwd, *range_inv, *corr_rid, *corr_perm;
   double *Wwp, *Wc;
   int *Wn;
@@ -479,12 +482,16 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
   const double w2p = 0.5 * (s - t - u);
   const double w2 = 0.5 * (s + t - u);
 
-  // phase C role: (point, site, pair)
-  const int ce = tid / (2 * L);
-  const int crem = tid - ce * 2 * L;
-  const int cpair = crem / L;
-  const int cj = crem - cpair * L;
-  const bool cactive = ce < nb;
+  // phase C role: (point, site, pair).  Lanes come in runs of 2 nb, one run
+  // per site, sites in decreasing order of their lattice-bubble entry counts:
+  // a wave then holds sites of similar cost, and its bubble loop runs about as
+  // long as its own lanes need (modelled at L = 19: 24 % fewer wave
+  // iterations than sites in index order)
+  const int crank = tid / (2 * nb);
+  const int crem = tid - crank * 2 * nb;
+  const int ce = crem >> 1, cpair = crem & 1;
+  const bool cactive = crank < L;
+  const int cj = cactive ? F.site_order[crank] : 0;
   double acc[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) acc[k] = 0.0;
@@ -1053,7 +1060,7 @@ int sp_tri_ctx_footprint(const sp_tri_problem *p, long long *bytes) {
   b += 3LL * p->nw * 8;                                                       // Sigma, dSigma, mesh
   b += 2 * (long long)p->nw * emax * 8 + 2 * emax * 8 + (p->nw + 1) * 4;      // weight tables
   b += emax * 10 * p->L * 8 + 10LL * p->B * p->Rb * 8 + 16;                   // correlation scratch, output, flags
-  b += 4LL * (8 * p->L + 1 + 5 * O + 2 * p->R + 4 * p->B * p->Rb);            // lattice tables
+  b += 4LL * (10 * p->L + 1 + 5 * O + 2 * p->R + 4 * p->B * p->Rb);           // lattice tables
   *bytes = b;
   return SP_OK;
 }
@@ -1099,8 +1106,10 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   const size_t nc = (size_t)f.B * f.Rb;
   double *mesh = nullptr;
   // lattice-bubble tables: per site, the overlap entries stably ordered by
-  // their spin transformations (P1, P2) and cut into groups of equal (P1, P2)
-  std::vector<int> gofs(f.L + 1, 0), ent;
+  // their spin transformations (P1, P2) and cut into groups of equal (P1, P2),
+  // the groups of a site by decreasing size (the lanes of a wave then run
+  // their longest groups side by side)
+  std::vector<int> gofs(f.L + 1, 0), ent, order(f.L);
   std::vector<int4> grp;
   ent.reserve(O);
   for (int r = 0; r < f.L; ++r) {
@@ -1126,12 +1135,19 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
       grp.back().y = (int)ent.size();
     }
     gofs[r + 1] = (int)grp.size();
+    std::stable_sort(grp.begin() + gofs[r], grp.end(),
+                     [](const int4 &a, const int4 &b) { return a.y - a.x > b.y - b.x; });
   }
+  for (int r = 0; r < f.L; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return p->ovl_off[a + 1] - p->ovl_off[a] > p->ovl_off[b + 1] - p->ovl_off[b];
+  });
   int *site = nullptr, *sitep = nullptr, *inv = nullptr, *invp = nullptr, *goff = nullptr, *bent = nullptr,
-      *rf = nullptr, *ri = nullptr, *corr = nullptr, *corrp = nullptr;
+      *rf = nullptr, *ri = nullptr, *corr = nullptr, *corrp = nullptr, *sorder = nullptr;
   int4 *bgrp = nullptr;
   if ((rc = talloc(ctx, &mesh, f.nw)) || (rc = talloc(ctx, &site, f.L)) || (rc = talloc(ctx, &sitep, 3 * f.L)) ||
       (rc = talloc(ctx, &inv, f.L)) || (rc = talloc(ctx, &invp, 3 * f.L)) || (rc = talloc(ctx, &goff, f.L + 1)) ||
+      (rc = talloc(ctx, &sorder, f.L)) ||
       (rc = talloc(ctx, &bent, ent.size())) || (rc = talloc(ctx, &bgrp, grp.size())) ||
       (rc = talloc(ctx, &rf, f.R)) || (rc = talloc(ctx, &ri, f.R)) ||
       (rc = talloc(ctx, &corr, nc)) || (rc = talloc(ctx, &corrp, 3 * nc)) || (rc = talloc(ctx, &f.G, nv)) ||
@@ -1148,6 +1164,7 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   f.inv_perm = invp;
   f.bgrp_off = goff;
   f.bent = bent;
+  f.site_order = sorder;
   f.bgrp = bgrp;
   f.range_fwd = rf;
   f.range_inv = ri;
@@ -1162,6 +1179,7 @@ int sp_tri_ctx_create(const sp_tri_problem *p, sp_tri_ctx **out) {
   SP_THIP(hipMemcpy(inv, p->inv_rid, f.L * 4, hipMemcpyHostToDevice));
   SP_THIP(hipMemcpy(invp, p->inv_perm, 3 * f.L * 4, hipMemcpyHostToDevice));
   SP_THIP(hipMemcpy(goff, gofs.data(), (f.L + 1) * 4, hipMemcpyHostToDevice));
+  SP_THIP(hipMemcpy(sorder, order.data(), f.L * 4, hipMemcpyHostToDevice));
   if (!ent.empty()) {
     SP_THIP(hipMemcpy(bent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice));
     SP_THIP(hipMemcpy(bgrp, grp.data(), grp.size() * sizeof(int4), hipMemcpyHostToDevice));
