@@ -23,8 +23,9 @@
 //
 // dGamma kernel: one workgroup per frequency triple (s, t, u), 128 threads for
 // lattices of at most 32 sites and 256 above, items walked t-major in XCD
-// runs.  Per channel the evaluation points are
-// processed in chunks of nb points:
+// runs; lattices of 13 to 32 sites run instantiations with the LDS bundle
+// geometry compiled in (every component offset an immediate).  Per channel
+// the evaluation points are processed in chunks of nb points:
 //   A  one thread per (point, access buffer) builds the descriptor (4 supports,
 //      weights, exchange / sign flags), the T channel's four site-local
 //      vertices included;
@@ -38,6 +39,8 @@
 //      transformations, so that a group's permutations become row addresses
 //      and one output permutation per group), the chalice and inverse
 //      chalice (256 products each), accumulated with the point's weight.
+//      Lanes take the sites in decreasing order of their overlap-entry
+//      counts, so that a wave's sites cost about the same.
 // The per-thread sums are reduced through LDS at the end of the item.
 #include <hip/hip_runtime.h>
 
