@@ -231,7 +231,7 @@ struct TriDev {
   int *flags;  // [0] NaN in the flow, [1] evaluation-list overflow
   int nw, L, R, B, Rb, emax, nb, euler;
   int nt;  // threads per workgroup of the dGamma kernel
-  int cs;  // component stride of its LDS bundles compiled into it (doubles); 0: L, read at run time
+  int cs;  // site stride of the LDS bundles compiled into the dGamma kernel (doubles); 0: run-time form
   long long nf, nv;
   unsigned g_bytes;
   double cut, step;
@@ -435,8 +435,8 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
   const int FS = 16 * L;
   // bundle geometry: compile-time constants for CS > 0, so that the
   // component offsets of every bundle read and write are immediates (the
-  // lattice bubble's 32 reads per overlap entry then need 6 address
-  // additions instead of 38)
+  // lattice bubble's 32 reads per overlap entry then need 9 address
+  // instructions instead of 38)
   const TriBundle geo = tri_bundle(L, CS, nb);
   const int RS = geo.rs, KS = geo.ks, BS = geo.bs;
   const int SS = CS > 0 ? CS : L;  // site stride of s_cf, s_ci
@@ -489,7 +489,7 @@ __global__ __launch_bounds__(NT, SPFRG_TRI_MIN_BLOCKS) void k_tri_dgamma(TriDev 
   // per site, sites in decreasing order of their lattice-bubble entry counts:
   // a wave then holds sites of similar cost, and its bubble loop runs about as
   // long as its own lanes need (modelled at L = 19: 24 % fewer wave
-  // iterations than sites in index order)
+  // iterations than sites in index order; measured 2.5 % faster)
   const int crank = tid / (2 * nb);
   const int crem = tid - crank * 2 * nb;
   const int ce = crem >> 1, cpair = crem & 1;
@@ -1005,6 +1005,10 @@ constexpr int tri_shape(int cs, int nb) { return cs * 32 + nb; }
 using TriShapes = std::integer_sequence<int, 0, tri_shape(16, 4), tri_shape(20, 3), tri_shape(24, 2), tri_shape(28, 2),
                                         tri_shape(32, 2)>;
 static_assert(kTriSmallL == 32, "TriShapes covers the lattices up to kTriSmallL sites");
+template <int... SH> constexpr bool tri_shapes_unpaired(std::integer_sequence<int, SH...>) {
+  return ((SH == 0 || tri_bundle(1, SH / 32, SH % 32).ks > 255) && ...);
+}
+static_assert(tri_shapes_unpaired(TriShapes{}), "the columns of a compiled shape must lie over 255 doubles apart");
 
 template <int... SH> constexpr bool tri_shape_compiled(int cs, int nb, std::integer_sequence<int, SH...>) {
   return ((SH != 0 && SH == tri_shape(cs, nb)) || ...);
