@@ -274,10 +274,11 @@ __global__ __launch_bounds__(256) void k_dsigma(const FlowDev *__restrict__ fl) 
 // LDS carve-up (bytes), computed identically on host and device.  Lp = L
 // rounded up to a multiple of 4 (register tiles of the lattice-bubble update); vs =
 // size of the gather type (8 FP64, 4 mixed).  The per-chunk descriptors (c,
-// off, w, ex, loc) are double-buffered: the producer wave fills one buffer
-// while the worker waves read the other.
+// off, wd, ex, loc) are double-buffered: the producer wave fills one buffer
+// while the worker waves read the other.  Only the signed density weights
+// are stored; the spin weights are their absolute values.
 struct LdsLayout {
-  int mesh, c, off, w, wd, ex, loc, dbuf, P, Q, C, rpa, red, site, inv, total;  // buffer b at + b * dbuf
+  int mesh, c, off, wd, ex, loc, dbuf, P, Q, C, rpa, red, site, inv, total;  // buffer b at + b * dbuf
 };
 
 __host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots, int vs) {
@@ -289,7 +290,6 @@ __host__ __device__ inline LdsLayout lds_layout(int nw, int L, int nb, int slots
   const int d0 = off;
   l.c = take(nb * 8);
   l.off = take(nb * 4 * 16);
-  l.w = take(nb * 4 * 4 * vs);
   l.wd = take(nb * 4 * 4 * vs);
   l.ex = take(nb * 4 * 4);
   l.loc = take(nb * 4 * 2 * vs);
@@ -434,7 +434,6 @@ __global__ __launch_bounds__(NT, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOC
           const double w2 = 0.5 * (s + t - u);
           double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
           int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
-          V2 *s_w = reinterpret_cast<V2 *>(lds_raw + lay.w + bf * lay.dbuf);
           V2 *s_wd = reinterpret_cast<V2 *>(lds_raw + lay.wd + bf * lay.dbuf);
           int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex + bf * lay.dbuf);
           V *s_loc = reinterpret_cast<V *>(lds_raw + lay.loc + bf * lay.dbuf);
@@ -472,8 +471,6 @@ __global__ __launch_bounds__(NT, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOC
               const Desc4 d = assemble(ch, xo, ia, ib, nw, rowbytes);
               const int sk = e * 4 + 2 * h + pr;
               s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
-              s_w[2 * sk] = mk2<V2>(V(d.ws[0]), V(d.ws[1]));
-              s_w[2 * sk + 1] = mk2<V2>(V(d.ws[2]), V(d.ws[3]));
               s_wd[2 * sk] = mk2<V2>(V(d.wd[0]), V(d.wd[1]));
               s_wd[2 * sk + 1] = mk2<V2>(V(d.wd[2]), V(d.wd[3]));
               s_ex[sk] = exb;
@@ -506,7 +503,6 @@ __global__ __launch_bounds__(NT, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOC
           const double w2 = 0.5 * (s + t - u);
           double *s_c = reinterpret_cast<double *>(lds_raw + lay.c + bf * lay.dbuf);
           int4 *s_off = reinterpret_cast<int4 *>(lds_raw + lay.off + bf * lay.dbuf);
-          V2 *s_w = reinterpret_cast<V2 *>(lds_raw + lay.w + bf * lay.dbuf);
           V2 *s_wd = reinterpret_cast<V2 *>(lds_raw + lay.wd + bf * lay.dbuf);
           int *s_ex = reinterpret_cast<int *>(lds_raw + lay.ex + bf * lay.dbuf);
           V *s_loc = reinterpret_cast<V *>(lds_raw + lay.loc + bf * lay.dbuf);
@@ -564,8 +560,6 @@ __global__ __launch_bounds__(NT, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOC
             if (bb.k < 4) {
               const int sk = e * 4 + bb.k;
               s_off[sk] = make_int4(d.off[0], d.off[1], d.off[2], d.off[3]);
-              s_w[2 * sk] = mk2<V2>(V(d.ws[0]), V(d.ws[1]));
-              s_w[2 * sk + 1] = mk2<V2>(V(d.ws[2]), V(d.ws[3]));
               s_wd[2 * sk] = mk2<V2>(V(d.wd[0]), V(d.wd[1]));
               s_wd[2 * sk + 1] = mk2<V2>(V(d.wd[2]), V(d.wd[3]));
               s_ex[sk] = ((bb.a < 0) != (bb.c < 0)) ? 1 : 0;
@@ -592,7 +586,6 @@ __global__ __launch_bounds__(NT, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOC
       // per iteration so that 16 U loads are in flight before the first use
       const double *s_c = reinterpret_cast<const double *>(lds_raw + lay.c + cur * lay.dbuf);
       const int4 *s_off = reinterpret_cast<const int4 *>(lds_raw + lay.off + cur * lay.dbuf);
-      const V2 *s_w = reinterpret_cast<const V2 *>(lds_raw + lay.w + cur * lay.dbuf);
       const V2 *s_wd = reinterpret_cast<const V2 *>(lds_raw + lay.wd + cur * lay.dbuf);
       const int *s_ex = reinterpret_cast<const int *>(lds_raw + lay.ex + cur * lay.dbuf);
       const V *s_loc = reinterpret_cast<const V *>(lds_raw + lay.loc + cur * lay.dbuf);
@@ -640,9 +633,12 @@ __global__ __launch_bounds__(NT, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOC
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
             const int sk = ee * 4 + k;
-            const V2 ws01 = s_w[2 * sk], ws23 = s_w[2 * sk + 1];
             // density weights carry the sign of the s <-> u exchanged storage
             const V2 wd01 = s_wd[2 * sk], wd23 = s_wd[2 * sk + 1];
+            // spin weight = |density weight|: the bilinear weights are >= 0, so
+            // the absolute value (a free source modifier of the FMA) is exact,
+            // and the workers read half the weight bytes from LDS
+            const V2 ws01 = mk2<V2>(fabs(wd01.x), fabs(wd01.y)), ws23 = mk2<V2>(fabs(wd23.x), fabs(wd23.y));
             const int exf = s_ex[sk];
             V2 v[4];
             if (kPre) {
