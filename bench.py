@@ -106,9 +106,9 @@ CORES = {
 # 2.00; WRITE_SIZE reads them exactly, scale 1.00).  The profile summaries
 # under profiles/ hold the runs and the commands.
 TRAFFIC_KIB = {
-    ("cubic-j1j2-r6-nw64", "f64"): {"fetch_kib": 774923.0, "write_kib": 153115.0,
+    ("cubic-j1j2-r6-nw64", "f64"): {"fetch_kib": 738093.0, "write_kib": 47871.7,
                                     "fetch_scale": 2.00, "write_scale": 1.00},
-    ("square-r4-nw32", "f64"): {"fetch_kib": 6936.99, "write_kib": 4518.58,
+    ("square-r4-nw32", "f64"): {"fetch_kib": 6045.33, "write_kib": 2376.01,
                                 "fetch_scale": 2.00, "write_scale": 1.00},
     ("kagome-dm-r5-nw48", "f64"): {"fetch_kib": 17787800.0, "write_kib": 134064.0,
                                    "fetch_scale": 2.00, "write_scale": 1.00},
