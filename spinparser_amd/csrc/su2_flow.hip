@@ -86,7 +86,7 @@ namespace {
 #define SPFRG_MIN_BLOCKS 4
 #endif
 #ifndef SPFRG_MIN_BLOCKS_MIXED
-#define SPFRG_MIN_BLOCKS_MIXED 4
+#define SPFRG_MIN_BLOCKS_MIXED 5
 #endif
 constexpr int kThreads = SPFRG_THREADS;
 // small lattices: the producer's work per evaluation point does not shrink
@@ -334,8 +334,8 @@ __device__ __forceinline__ void ksum_add(KSum &k, float x) {
 // and evaluate chunk j, the producer builds the descriptors of chunk j + 1 into
 // the other LDS buffer, so the bracket searches and the t channel's site-local
 // gathers overlap the workers' gathers instead of stalling them.
-template <bool MIXED, int NT>
-__global__ __launch_bounds__(NT, MIXED ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOCKS) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,
+template <bool MIXED, int NT, int MINB>
+__global__ __launch_bounds__(NT, MINB) void k_dgamma(const FlowDev *__restrict__ fl, int nflows, long long total,
                                                      long long per_xcd) {
   using V = typename std::conditional<MIXED, float, double>::type;
   using V2 = typename std::conditional<MIXED, float2, double2>::type;
@@ -1186,27 +1186,46 @@ int check_problem(sp_ctx *ctx, const sp_su2_problem *p) {
   return SP_OK;
 }
 
-// LDS budget of one dGamma workgroup: a quarter of the CU's 160 KB, which
-// keeps four 256-thread workgroups (four waves per SIMD) resident.  The
-// 128-thread workgroups of small lattices keep it too: more points per chunk
-// at six or seven workgroups per CU measured 2-6 % faster than the eighth
-// of the LDS that eight resident workgroups would allow (L = 9, 11, 16)
+// Workgroups per CU that the registers of a dGamma kernel allow: one wave per
+// SIMD each for the 256-thread shape, so SPFRG_MIN_BLOCKS (FP64, 4) or
+// SPFRG_MIN_BLOCKS_MIXED (mixed form, 5: it needs fewer registers; the mixed
+// 256-thread kernel is also compiled for 4, which flows whose LDS plan needs
+// more than 160 KB / 5 run, since its code is 1-3 % faster at L = 31).  The
+// 128-thread shape of small lattices is compiled for 4 in both forms.
+constexpr int resident_blocks(int vs, int nt) {
+  return (vs == 4 && nt != kSmallThreads) ? SPFRG_MIN_BLOCKS_MIXED : SPFRG_MIN_BLOCKS;
+}
+// LDS budget of one dGamma workgroup: a quarter of the CU's 160 KB keeps four
+// workgroups resident.  The 128-thread workgroups of small lattices keep it
+// too: more points per chunk at six or seven workgroups per CU measured 2-6 %
+// faster than the eighth of the LDS that eight resident workgroups would allow
+// (L = 9, 11, 16).  A kernel whose registers allow more workgroups per CU
+// takes the smaller share 160 KB / resident_blocks when its chunks still hold
+// at least three points per worker slot (mixed form, L = 23: 24 instead of 32
+// points, 3-14 % faster); with fewer (L = 31: 12 instead of 24) the larger
+// chunks at four workgroups per CU were faster at small cutoffs.
 constexpr size_t kLdsBudget = 160 * 1024 / 4;
+constexpr int kMinPerSlotHigh = 3;
 
 // worker slots (all waves but the last) and points per chunk of a flow whose
 // dGamma workgroups have nt threads; the producer wave holds one lane per
 // (point, pair), so nb <= 32 (points per slot and chunk shrink while the LDS
-// plan exceeds kLdsBudget)
+// plan exceeds the budget)
 void launch_shape(int nw, int L, int vs, int nt, int &slots, int &nb, size_t &lds) {
   slots = (nt - 64) / L;
-  int per = (vs == 4 ? kMaxBatchMixed : kMaxBatch) / slots;
-  if (per < 1) per = 1;
-  for (;;) {
-    nb = slots * per;
-    if (nb > 32) nb = 32;
+  const int per0 = (vs == 4 ? kMaxBatchMixed : kMaxBatch) / slots > 1 ? (vs == 4 ? kMaxBatchMixed : kMaxBatch) / slots : 1;
+  const int rb = resident_blocks(vs, nt);
+  if (rb > 4) {
+    for (int per = per0; per >= kMinPerSlotHigh; --per) {
+      nb = slots * per < 32 ? slots * per : 32;
+      lds = (size_t)lds_layout(nw, L, nb, slots, vs).total;
+      if (lds <= (size_t)160 * 1024 / rb) return;
+    }
+  }
+  for (int per = per0;; --per) {
+    nb = slots * per < 32 ? slots * per : 32;
     lds = (size_t)lds_layout(nw, L, nb, slots, vs).total;
     if (lds <= kLdsBudget || per == 1) break;
-    --per;
   }
 }
 
@@ -1215,13 +1234,16 @@ size_t lds_of(const FlowDev &f) {
 }
 
 // Flows of one dGamma launch share its workgroup size and its dynamic LDS
-// size, the largest of them; a flow whose plan fits kLdsBudget must not run at
+// size, the largest of them; a flow whose plan fits a budget must not run at
 // the occupancy of a larger one.  Launch class = precision x workgroup size x
-// (LDS plan within the budget or not); a batch issues one launch per class.
-constexpr int kLaunchClasses = 8;
+// LDS level (within 160 KB / resident_blocks, within kLdsBudget, above); a
+// batch issues one launch per class.
+constexpr int kLaunchClasses = 12;
 int launch_class(const FlowDev &f) {
-  return (f.precision == SP_PRECISION_MIXED ? 2 : 0) + (lds_of(f) > kLdsBudget ? 1 : 0) +
-         (f.nt == kSmallThreads ? 4 : 0);
+  const int vs = f.precision == SP_PRECISION_MIXED ? 4 : 8;
+  const size_t lds = lds_of(f);
+  const int level = lds <= (size_t)160 * 1024 / resident_blocks(vs, f.nt) ? 0 : (lds <= kLdsBudget ? 1 : 2);
+  return ((vs == 4 ? 2 : 0) + (f.nt == kSmallThreads ? 1 : 0)) * 3 + level;
 }
 
 // Launch geometry of a set of flows held in a host array (h) whose device copy is d.
@@ -1272,18 +1294,23 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
       const long long rounds = (total + 8 * kChunk - 1) / (8 * kChunk);
       const unsigned grid = (unsigned)(rounds * 8 * kChunk);
       const int nt = h[i0].nt;
+      // LDS level 0 of a kernel compiled for more resident workgroups
+      const bool high = cls % 3 == 0 && resident_blocks(mixed ? 4 : 8, nt) > SPFRG_MIN_BLOCKS;
       if (mixed && nt == kSmallThreads)
-        hipLaunchKernelGGL((k_dgamma<true, kSmallThreads>), dim3(grid), dim3(nt), lds, st, d + i0, i1 - i0, total,
-                           per_xcd);
+        hipLaunchKernelGGL((k_dgamma<true, kSmallThreads, SPFRG_MIN_BLOCKS>), dim3(grid), dim3(nt), lds, st, d + i0,
+                           i1 - i0, total, per_xcd);
+      else if (mixed && high)
+        hipLaunchKernelGGL((k_dgamma<true, kThreads, SPFRG_MIN_BLOCKS_MIXED>), dim3(grid), dim3(nt), lds, st, d + i0,
+                           i1 - i0, total, per_xcd);
       else if (mixed)
-        hipLaunchKernelGGL((k_dgamma<true, kThreads>), dim3(grid), dim3(nt), lds, st, d + i0, i1 - i0, total,
-                           per_xcd);
+        hipLaunchKernelGGL((k_dgamma<true, kThreads, SPFRG_MIN_BLOCKS>), dim3(grid), dim3(nt), lds, st, d + i0,
+                           i1 - i0, total, per_xcd);
       else if (nt == kSmallThreads)
-        hipLaunchKernelGGL((k_dgamma<false, kSmallThreads>), dim3(grid), dim3(nt), lds, st, d + i0, i1 - i0, total,
-                           per_xcd);
+        hipLaunchKernelGGL((k_dgamma<false, kSmallThreads, SPFRG_MIN_BLOCKS>), dim3(grid), dim3(nt), lds, st, d + i0,
+                           i1 - i0, total, per_xcd);
       else
-        hipLaunchKernelGGL((k_dgamma<false, kThreads>), dim3(grid), dim3(nt), lds, st, d + i0, i1 - i0, total,
-                           per_xcd);
+        hipLaunchKernelGGL((k_dgamma<false, kThreads, SPFRG_MIN_BLOCKS>), dim3(grid), dim3(nt), lds, st, d + i0,
+                           i1 - i0, total, per_xcd);
       e = hipGetLastError();
       if (e != hipSuccess) return e;
     }
@@ -1464,14 +1491,16 @@ int sp_ctx_create(const sp_su2_problem *p, sp_ctx **out) {
   SP_HIP(hipMemset(f.flags, 0, 32));
   SP_HIP(hipMemset(f.Wn, 0, f.nw * 4));
   SP_HIP(hipMemset(f.Cn, 0, 4));
-  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false, kThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             max_lds));
-  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true, kThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             max_lds));
-  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false, kSmallThreads>,
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false, kThreads, SPFRG_MIN_BLOCKS>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true, kSmallThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true, kThreads, SPFRG_MIN_BLOCKS>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true, kThreads, SPFRG_MIN_BLOCKS_MIXED>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<false, kSmallThreads, SPFRG_MIN_BLOCKS>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  SP_HIP(hipFuncSetAttribute((const void *)k_dgamma<true, kSmallThreads, SPFRG_MIN_BLOCKS>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   SP_HIP(hipMemcpy(ctx->d_fd, &f, sizeof(FlowDev), hipMemcpyHostToDevice));
   return SP_OK;
 }
@@ -1940,10 +1969,10 @@ int noalloc_scratch(int dev, unsigned char **base, int *max_lds) {
     if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return SP_EHIP;
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return SP_EHIP;
-    if (hipFuncSetAttribute((const void *)k_dgamma<false, kThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            lds) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_dgamma<false, kSmallThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            lds) != hipSuccess)
+    if (hipFuncSetAttribute((const void *)k_dgamma<false, kThreads, SPFRG_MIN_BLOCKS>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_dgamma<false, kSmallThreads, SPFRG_MIN_BLOCKS>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
       return SP_EHIP;
     g_noalloc_lds[dev] = lds;
     g_noalloc_ptr[dev] = static_cast<unsigned char *>(q);
