@@ -74,8 +74,9 @@ namespace {
 #ifndef SPFRG_THREADS
 #define SPFRG_THREADS 256
 #endif
-// evaluation points per chunk (FP64 / mixed form): the descriptor buffers and
-// bubble factors of a chunk must leave LDS for 4 workgroups per CU
+// evaluation points per chunk (FP64 / mixed form), at most: the descriptor
+// buffers and bubble factors of a chunk must leave LDS for the workgroups per
+// CU that the kernel's registers allow (launch_shape shrinks the chunk)
 #ifndef SPFRG_MAX_BATCH
 #define SPFRG_MAX_BATCH 16
 #endif
