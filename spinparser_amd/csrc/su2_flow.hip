@@ -1200,11 +1200,13 @@ constexpr int resident_blocks(int vs, int nt) {
 // workgroups resident.  The 128-thread workgroups of small lattices keep it
 // too: more points per chunk at six or seven workgroups per CU measured 2-6 %
 // faster than the eighth of the LDS that eight resident workgroups would allow
-// (L = 9, 11, 16).  A kernel whose registers allow more workgroups per CU
-// takes the smaller share 160 KB / resident_blocks when its chunks still hold
-// at least three points per worker slot (mixed form, L = 23: 24 instead of 32
-// points, 3-14 % faster); with fewer (L = 31: 12 instead of 24) the larger
-// chunks at four workgroups per CU were faster at small cutoffs.
+// (L = 9, 11, 16).  Where the registers allow more workgroups per CU, the
+// plan takes the smaller share 160 KB / resident_blocks if its chunks still
+// hold at least three points per worker slot (mixed form, L = 23: 24 instead
+// of 32 points, 3-14 % faster); with fewer (L = 31: 12 instead of 24) the
+// larger chunks at four workgroups per CU were faster at small cutoffs.  A
+// flow whose plan ends up within the smaller share runs the kernel compiled
+// for more workgroups, any other the one compiled for four.
 constexpr size_t kLdsBudget = 160 * 1024 / 4;
 constexpr int kMinPerSlotHigh = 3;
 
@@ -1214,7 +1216,8 @@ constexpr int kMinPerSlotHigh = 3;
 // plan exceeds the budget)
 void launch_shape(int nw, int L, int vs, int nt, int &slots, int &nb, size_t &lds) {
   slots = (nt - 64) / L;
-  const int per0 = (vs == 4 ? kMaxBatchMixed : kMaxBatch) / slots > 1 ? (vs == 4 ? kMaxBatchMixed : kMaxBatch) / slots : 1;
+  const int maxb = vs == 4 ? kMaxBatchMixed : kMaxBatch;
+  const int per0 = maxb / slots > 1 ? maxb / slots : 1;
   const int rb = resident_blocks(vs, nt);
   if (rb > 4) {
     for (int per = per0; per >= kMinPerSlotHigh; --per) {
@@ -1240,11 +1243,13 @@ size_t lds_of(const FlowDev &f) {
 // LDS level (within 160 KB / resident_blocks, within kLdsBudget, above); a
 // batch issues one launch per class.
 constexpr int kLaunchClasses = 12;
-int launch_class(const FlowDev &f) {
+int lds_level(const FlowDev &f) {
   const int vs = f.precision == SP_PRECISION_MIXED ? 4 : 8;
   const size_t lds = lds_of(f);
-  const int level = lds <= (size_t)160 * 1024 / resident_blocks(vs, f.nt) ? 0 : (lds <= kLdsBudget ? 1 : 2);
-  return ((vs == 4 ? 2 : 0) + (f.nt == kSmallThreads ? 1 : 0)) * 3 + level;
+  return lds <= (size_t)160 * 1024 / resident_blocks(vs, f.nt) ? 0 : (lds <= kLdsBudget ? 1 : 2);
+}
+int launch_class(const FlowDev &f) {
+  return ((f.precision == SP_PRECISION_MIXED ? 2 : 0) + (f.nt == kSmallThreads ? 1 : 0)) * 3 + lds_level(f);
 }
 
 // Launch geometry of a set of flows held in a host array (h) whose device copy is d.
@@ -1295,8 +1300,9 @@ hipError_t launch_compute(hipStream_t st, const FlowDev *d, const FlowDev *h, in
       const long long rounds = (total + 8 * kChunk - 1) / (8 * kChunk);
       const unsigned grid = (unsigned)(rounds * 8 * kChunk);
       const int nt = h[i0].nt;
-      // LDS level 0 of a kernel compiled for more resident workgroups
-      const bool high = cls % 3 == 0 && resident_blocks(mixed ? 4 : 8, nt) > SPFRG_MIN_BLOCKS;
+      // plans within 160 KB / resident_blocks run the kernel compiled for
+      // that many resident workgroups
+      const bool high = lds_level(h[i0]) == 0 && resident_blocks(mixed ? 4 : 8, nt) > SPFRG_MIN_BLOCKS;
       if (mixed && nt == kSmallThreads)
         hipLaunchKernelGGL((k_dgamma<true, kSmallThreads, SPFRG_MIN_BLOCKS>), dim3(grid), dim3(nt), lds, st, d + i0,
                            i1 - i0, total, per_xcd);
