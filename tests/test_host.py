@@ -185,6 +185,29 @@ def test_bench_stratified_sampling_and_estimator():
     assert abs(bench.full_schedule_seconds(flops, ks, t) - want) <= 1e-9 * want
 
 
+def test_bench_traffic_constants_match_profiles():
+    """The roofline traffic of the bench line comes from the FETCH_SIZE and
+    WRITE_SIZE constants in bench.py; each must be the figure that the
+    committed rocprofv3 summary of the same kernel and workload reports (the
+    first row of its HBM traffic table, KiB per dispatch, rounded there)."""
+    import re
+    import bench
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    files = {("cubic-j1j2-r6-nw64", "f64"): "k_dgamma_cubic-j1j2-r6-nw64_fp64.md",
+             ("square-r4-nw32", "f64"): "k_dgamma_square-r4-nw32_fp64.md",
+             ("kagome-dm-r5-nw48", "f64"): "k_tri_dgamma_kagome-dm-r5-nw48_fp64.md",
+             ("honeycomb-kitaev-r8-nw64", "f64"): "k_xyz_dgamma_honeycomb-kitaev-r8-nw64_fp64.md"}
+    assert set(files) == set(bench.TRAFFIC_KIB)
+    for key, name in files.items():
+        text = open(os.path.join(root, "profiles", name)).read()
+        head = text.index("| kernel | FETCH_SIZE | WRITE_SIZE |")
+        row = text[head:].split("\n")[2]
+        fetch, write = (float(x.replace(",", "")) for x in re.findall(r"\| ([0-9][0-9,.]*) ", row + " ")[:2])
+        t = bench.TRAFFIC_KIB[key]
+        assert abs(t["fetch_kib"] - fetch) <= max(1.0, 1e-4 * fetch), (name, t["fetch_kib"], fetch)
+        assert abs(t["write_kib"] - write) <= max(1.0, 1e-4 * write), (name, t["write_kib"], write)
+
+
 def test_bench_refuses_several_gpus():
     """bench.py measures one GPU: --gpus N > 1 exits with a message before any GPU work."""
     import subprocess
