@@ -110,7 +110,7 @@ TRAFFIC_KIB = {
                                     "fetch_scale": 2.00, "write_scale": 1.00},
     ("square-r4-nw32", "f64"): {"fetch_kib": 6045.33, "write_kib": 2376.01,
                                 "fetch_scale": 2.00, "write_scale": 1.00},
-    ("kagome-dm-r5-nw48", "f64"): {"fetch_kib": 17787800.0, "write_kib": 134064.0,
+    ("kagome-dm-r5-nw48", "f64"): {"fetch_kib": 17817400.0, "write_kib": 134064.0,
                                    "fetch_scale": 2.00, "write_scale": 1.00},
     ("honeycomb-kitaev-r8-nw64", "f64"): {"fetch_kib": 3350080.0, "write_kib": 96774.9,
                                           "fetch_scale": 2.00, "write_scale": 1.00},
