@@ -1208,7 +1208,10 @@ constexpr int resident_blocks(int vs, int nt) {
 // flow whose plan ends up within the smaller share runs the kernel compiled
 // for more workgroups, any other the one compiled for four.
 constexpr size_t kLdsBudget = 160 * 1024 / 4;
-constexpr int kMinPerSlotHigh = 3;
+#ifndef SPFRG_MIN_PER_SLOT_HIGH
+#define SPFRG_MIN_PER_SLOT_HIGH 3
+#endif
+constexpr int kMinPerSlotHigh = SPFRG_MIN_PER_SLOT_HIGH;
 
 // worker slots (all waves but the last) and points per chunk of a flow whose
 // dGamma workgroups have nt threads; the producer wave holds one lane per
